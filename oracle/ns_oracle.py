@@ -1,0 +1,300 @@
+"""CPU oracle for the single-device Chorin projection step and rollout.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
+module; only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg use it, and only as the checker (or as the timed CPU
+baseline), never as the code path being measured or shipped.
+
+A NumPy restatement of the reference's single-core solver
+(``NS_solver.py`` of helmhlotz/ns_solver_jax_differential_distributed).
+Every function runs in one of two precisions:
+
+* ``dtype=np.float32`` follows the reference's own arithmetic (JAX with x64
+  off): f32 arrays, Python-float coefficients folded in as f32, complex64
+  FFTs, f32 objective accumulation.
+* ``dtype=np.float64`` is the FP64 accuracy oracle.
+
+Each function cites the reference line range it restates.
+
+Parity pinning (each item is checked in ``tests/test_oracle.py``):
+
+* rollout / forcing / J: reference result files F1..F5 (``tests/golden``),
+  the mean-mode closed form C1, and JAX's own reverse-mode gradient
+  recovered from the committed SGD trajectory (F5);
+* stencils: Fourier-symbol closed forms (C4) and the reference's
+  second-order convergence test;
+* Poisson: manufactured solution (reference ``unit_tests`` 372-416) and a
+  dense linear-algebra solve on tiny grids;
+* projection: single-mode symbol ``I - s s^T/|k|^2`` (C2);
+* adjoint: central finite differences of the FP64 rollout, dot-product
+  tests and a brute-force dense Jacobian on an 8x8 grid (C6).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+TWO_PI = 2.0 * math.pi
+
+
+def _f(dtype, v):
+    """A Python float folded into ``dtype`` (JAX weak-type promotion)."""
+    return dtype.type(v) if isinstance(dtype, np.dtype) else np.dtype(dtype).type(v)
+
+
+# --------------------------------------------------------------------------
+# Config (NS_solver.py:66-79, 112-117)
+# --------------------------------------------------------------------------
+def config_dt(N: int, L: float = 1.0, nu: float = 0.01, cfl: float = 0.4,
+              v_max: float = 1.0) -> float:
+    """``SimConfig.dt``: f32 ``jnp.minimum`` of the CFL and diffusive limits
+    converted back to a Python float (NS_solver.py:74-79)."""
+    dx = L / N
+    dt_adv = cfl * dx / v_max
+    dt_diff = dx ** 2 / (4.0 * nu)
+    return float(np.minimum(np.float32(dt_adv), np.float32(dt_diff)))
+
+
+def config_steps(t_end: float, dt: float) -> int:
+    """``SimConfig.steps`` = ceil(t/dt) (NS_solver.py:66-68)."""
+    return int(np.ceil(t_end / dt))
+
+
+def grid_x(N: int, L: float = 1.0, dtype=np.float64) -> np.ndarray:
+    """1-D coordinates of ``make_grid`` (NS_solver.py:112-117):
+    ``linspace(0, L, N, endpoint=False)``; X varies along axis 0 (``ij``)."""
+    dtype = np.dtype(dtype)
+    frac = (np.arange(N, dtype=np.float64) / N).astype(dtype)
+    return (_f(dtype, L) * frac).astype(dtype)
+
+
+# --------------------------------------------------------------------------
+# Stencils (NS_solver.py:206-210, 243-265, 468-477)
+# --------------------------------------------------------------------------
+def grad(field: np.ndarray, dx: float):
+    """Central gradient with periodic wrap (NS_solver.py:243-247)."""
+    dt_ = field.dtype
+    two_dx = _f(dt_, 2 * dx)
+    gx = (np.roll(field, -1, axis=0) - np.roll(field, 1, axis=0)) / two_dx
+    gy = (np.roll(field, -1, axis=1) - np.roll(field, 1, axis=1)) / two_dx
+    return gx, gy
+
+
+def laplacian(field: np.ndarray, dx: float) -> np.ndarray:
+    """5-point Laplacian (NS_solver.py:249-255)."""
+    dt_ = field.dtype
+    left = np.roll(field, 1, axis=0)
+    right = np.roll(field, -1, axis=0)
+    up = np.roll(field, -1, axis=1)
+    down = np.roll(field, 1, axis=1)
+    return (left + right + up + down - _f(dt_, 4) * field) / _f(dt_, dx ** 2)
+
+
+def upwind_advect(field, ux, uy, dx: float) -> np.ndarray:
+    """First-order upwind -(u.grad)phi; backward difference where u >= 0
+    (NS_solver.py:257-265)."""
+    dt_ = field.dtype
+    f_left = np.roll(field, 1, axis=0)
+    f_right = np.roll(field, -1, axis=0)
+    dfx = np.where(ux >= 0, field - f_left, f_right - field) / _f(dt_, dx)
+    f_down = np.roll(field, 1, axis=1)
+    f_up = np.roll(field, -1, axis=1)
+    dfy = np.where(uy >= 0, field - f_down, f_up - field) / _f(dt_, dx)
+    return -(ux * dfx + uy * dfy)
+
+
+def central_advect(field, ux, uy, dx: float) -> np.ndarray:
+    """``CentralAdvection.advect`` = -(ux d_x phi + uy d_y phi)
+    (NS_solver.py:474-477)."""
+    fx, fy = grad(field, dx)
+    return -(ux * fx + uy * fy)
+
+
+def advect(field, ux, uy, dx: float, scheme: str) -> np.ndarray:
+    """``make_advection_scheme(name).advect`` (NS_solver.py:468-497)."""
+    if scheme == "central":
+        return central_advect(field, ux, uy, dx)
+    if scheme == "upwind":
+        return upwind_advect(field, ux, uy, dx)
+    raise ValueError(f"Unknown advection scheme: '{scheme}'")
+
+
+def divergence(u: np.ndarray, dx: float) -> np.ndarray:
+    """d_x u_x + d_y u_y via ``grad`` (NS_solver.py:206-210)."""
+    gx, _ = grad(u[0], dx)
+    _, gy = grad(u[1], dx)
+    return gx + gy
+
+
+# --------------------------------------------------------------------------
+# Spectral Poisson (NS_solver.py:517-527)
+# --------------------------------------------------------------------------
+def inv_k2(N: int, dx: float, dtype=np.float64) -> np.ndarray:
+    """``SpectralPoissonSolver._inv_k2`` with the k=0 entry set to 0
+    (NS_solver.py:517-521)."""
+    dtype = np.dtype(dtype)
+    k = (np.fft.fftfreq(N, d=dx) * 2 * np.pi).astype(dtype)
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    k2 = KX ** 2 + KY ** 2
+    with np.errstate(divide="ignore"):
+        out = np.where(k2 == 0, dtype.type(0), dtype.type(1) / np.where(k2 == 0, 1, k2))
+    return out.astype(dtype)
+
+
+def poisson(rhs: np.ndarray, dx: float) -> np.ndarray:
+    """p = ifft2(-fft2(rhs) / |k|^2).real with p_hat(0,0)=0, i.e. lap p = rhs
+    (NS_solver.py:523-527)."""
+    N = rhs.shape[0]
+    ik2 = inv_k2(N, dx, rhs.dtype)
+    p_hat = -np.fft.fft2(rhs) * ik2
+    p_hat[0, 0] = 0.0
+    return np.fft.ifft2(p_hat).real.astype(rhs.dtype)
+
+
+# --------------------------------------------------------------------------
+# Forcing (NS_solver.py:621-627, 637-640)
+# --------------------------------------------------------------------------
+@dataclass(frozen=True)
+class PhysParams:
+    """``Phys`` (NS_solver.py:95-101)."""
+    nu: float
+    mu0: float = 0.5
+    sigma0: float = 0.1
+    C: float = 1.0
+
+
+def forcing_x(theta, t, x: np.ndarray, phys: PhysParams) -> np.ndarray:
+    """``GaussianForcingField._x_component`` (NS_solver.py:621-627).
+
+    Only theta[0] (A) and theta[1] (B) are read; the wrap ``d - round(d)``
+    has period 1 whatever L is, and ``round`` is half-to-even."""
+    dtype = x.dtype
+    A = dtype.type(theta[0])
+    B = dtype.type(theta[1])
+    t = dtype.type(t)
+    two_pi = _f(dtype, 2.0 * np.pi)
+    mu_t = _f(dtype, phys.mu0) + np.sin(two_pi * A * t)
+    sig_t = _f(dtype, phys.sigma0) + np.cos(two_pi * B * t) ** 2
+    d = (x - mu_t) - np.round(x - mu_t)
+    return (_f(dtype, phys.C) * np.exp(-(d * d) / (_f(dtype, 2.0) * sig_t * sig_t))).astype(dtype)
+
+
+def forcing(theta, t, N: int, L: float, phys: PhysParams, dtype=np.float64) -> np.ndarray:
+    """``GaussianForcingField.__call__``: (2,N,N), f_y = 0, f_x depends on
+    x (axis 0) only (NS_solver.py:637-640)."""
+    x = grid_x(N, L, dtype)
+    fx = forcing_x(theta, t, x, phys)
+    f = np.zeros((2, N, N), dtype=dtype)
+    f[0] = fx[:, None]
+    return f
+
+
+def step_time(i: int, dt: float, dtype=np.float64):
+    """t = i*dt as the scan computes it: int32 index times a weak float
+    (NS_solver.py:744)."""
+    dtype = np.dtype(dtype)
+    return dtype.type(i) * dtype.type(dt)
+
+
+# --------------------------------------------------------------------------
+# Chorin step (NS_solver.py:671-697)
+# --------------------------------------------------------------------------
+def u_star(u, f, dx, nu, dt, scheme):
+    """Explicit-Euler advection-diffusion predictor (NS_solver.py:685-689)."""
+    dtype = u.dtype
+    ux, uy = u[0], u[1]
+    lap = np.stack([laplacian(ux, dx), laplacian(uy, dx)])
+    adv = np.stack([advect(ux, ux, uy, dx, scheme), advect(uy, ux, uy, dx, scheme)])
+    nu_ = _f(dtype, nu)
+    dt_ = _f(dtype, dt)
+    return np.stack([ux + dt_ * (nu_ * lap[0] + adv[0] + f[0]),
+                     uy + dt_ * (nu_ * lap[1] + adv[1] + f[1])])
+
+
+def project(us, dx, dt):
+    """Pressure solve and projection (NS_solver.py:691-697)."""
+    dtype = us.dtype
+    dt_ = _f(dtype, dt)
+    div = divergence(us, dx)
+    p = poisson(div / dt_, dx)
+    px, py = grad(p, dx)
+    return np.stack([us[0] - dt_ * px, us[1] - dt_ * py])
+
+
+def step(u, f, dx, nu, dt, scheme="central"):
+    """``ChorinProjector.step`` (NS_solver.py:671-697)."""
+    return project(u_star(u, f, dx, nu, dt, scheme), dx, dt)
+
+
+# --------------------------------------------------------------------------
+# Rollout and objective (NS_solver.py:732-756)
+# --------------------------------------------------------------------------
+def rollout(u0, theta, N, L, phys: PhysParams, dt, steps, scheme="central",
+            objective="work", keep_states=False):
+    """``SingleCoreNSSolver.rollout`` (NS_solver.py:732-750).
+
+    J accumulates *before* each step with t = i*dt.  ``objective="work"`` is
+    the reference's J = sum_i dt*sum(u_i*f_i)*dx^2.  ``objective="mean_ke"``
+    is the mean kinetic energy (1/steps)*sum_i mean(0.5|u_i|^2) over the same
+    pre-step states (not in the reference; see the module docstring of the
+    package).  Returns (J, u_final[, states])."""
+    dtype = np.asarray(u0).dtype
+    dx = L / N
+    x = grid_x(N, L, dtype)
+    u = np.array(u0, dtype=dtype, copy=True)
+    J = dtype.type(0)
+    states = [] if keep_states else None
+    dtf = _f(dtype, dt)
+    dx2 = _f(dtype, dx ** 2)
+    for i in range(steps):
+        t = step_time(i, dt, dtype)
+        fx = forcing_x(theta, t, x, phys)
+        f = np.zeros_like(u)
+        f[0] = fx[:, None]
+        if keep_states:
+            states.append(u.copy())
+        if objective == "work":
+            J = dtype.type(J + dtf * np.sum(u * f, dtype=dtype) * dx2)
+        elif objective == "mean_ke":
+            J = dtype.type(J + np.mean(0.5 * (u[0] ** 2 + u[1] ** 2), dtype=np.float64) / steps)
+        else:
+            raise ValueError(f"unknown objective {objective!r}")
+        u = step(u, f, dx, phys.nu, dt, scheme)
+    if keep_states:
+        return J, u, states
+    return J, u
+
+
+def objective(theta, N, L, phys, dt, steps, scheme="central", dtype=np.float64,
+              objective_kind="work"):
+    """``SingleCoreNSSolver.objective``: J(theta) from u0 = 0
+    (NS_solver.py:752-756)."""
+    u0 = np.zeros((2, N, N), dtype=dtype)
+    J, _ = rollout(u0, theta, N, L, phys, dt, steps, scheme, objective_kind)
+    return J
+
+
+def fd_gradient(theta, eps, J_fn):
+    """Central finite differences, 2*dim objective calls per gradient
+    (studies.py:306-318)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    g = np.zeros_like(theta)
+    for i in range(len(theta)):
+        tp = theta.copy()
+        tm = theta.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        g[i] = (J_fn(tp) - J_fn(tm)) / (2 * eps)
+    return g
+
+
+def kinetic_energy(u, dx) -> float:
+    """0.5*sum|u|^2*dx^2 (studies.py:76-77, 590)."""
+    return float(0.5 * np.sum(u[0].astype(np.float64) ** 2 + u[1].astype(np.float64) ** 2) * dx ** 2)
+
+
+def speed(u) -> np.ndarray:
+    """|u| as the runner saves it (runner.py:125)."""
+    return np.sqrt(u[0] ** 2 + u[1] ** 2)
