@@ -1,0 +1,181 @@
+/*
+ * nsk.h - C ABI of libnsk.so, the MI355X (gfx950) kernels behind the
+ * single-device Chorin projection step and rollout of
+ * helmhlotz/ns_solver_jax_differential_distributed (NS_solver.py).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative NSK_E_* code; the
+ *    message of the last failure on the calling thread is nsk_last_error().
+ *  - All pointers are device pointers unless stated; the caller owns every
+ *    buffer (state, spectra, partial sums, checkpoints).  The library never
+ *    allocates device memory.
+ *  - Real fields are float32, row-major (component, x, y) with y contiguous,
+ *    exactly the reference's (2, N, N) arrays; a batch of B independent
+ *    fields is B such blocks back to back.
+ *  - "Spectra" are the row-spectral form of a pair of real fields (a, b):
+ *    per row x, the half spectrum along y of a and of b, stored as
+ *    float4 {Re A, Im A, Re B, Im B} for ky = 1..N/2-1, with ky = 0 holding
+ *    {A(0), A(N/2), B(0), B(N/2)} (both real).  One spectrum buffer is
+ *    N*N/2 float4 = 8*N*N bytes per field pair.
+ *  - N must be a power of two with 8 <= N <= 4096 (NSK_E_UNSUPPORTED_N
+ *    otherwise).
+ *  - Work is enqueued on the given stream; nothing synchronises.
+ */
+#ifndef NSK_H
+#define NSK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSK_OK 0
+#define NSK_E_INVAL (-1)
+#define NSK_E_UNSUPPORTED_N (-2)
+#define NSK_E_HIP (-3)
+#define NSK_E_WORKSPACE (-4)
+
+#define NSK_SCHEME_CENTRAL 0 /* CentralAdvection, NS_solver.py:468-477 */
+#define NSK_SCHEME_UPWIND 1  /* UpwindAdvection,  NS_solver.py:480-488 */
+
+#define NSK_OBJ_WORK 0    /* J = sum_i dt*sum(u_i*f_i)*dx^2, NS_solver.py:746 */
+#define NSK_OBJ_MEAN_KE 1 /* J = (1/steps) sum_i mean(|u_i|^2/2) (no reference counterpart) */
+
+/* Grid and physics constants shared by every slot of a launch
+ * (Grid, Phys: NS_solver.py:86-101). */
+typedef struct nsk_plan {
+  int N;          /* grid points per side */
+  int batch;      /* number of independent slots */
+  double L;       /* domain length (make_grid, NS_solver.py:112-117) */
+  double dx;      /* L / N */
+  float Lf;       /* float(L): coordinates are Lf * (i / N) as jnp.linspace */
+  float mu0;      /* forcing centre offset (Phys.mu0) */
+  float sigma0;   /* forcing base width (Phys.sigma0) */
+  float C;        /* forcing amplitude (Phys.C) */
+  float inv_dx;   /* float(1/dx) */
+  float inv_2dx;  /* float(1/(2 dx)) */
+  float inv_dx2;  /* float(1/dx^2) */
+  float kscale;   /* float((2 pi / L)^2): |k|^2 = kscale * (i^2 + j^2) */
+} nsk_plan;
+
+/* Per-slot state, one per trajectory.  theta = (A, B) as read by
+ * GaussianForcingField._x_component (NS_solver.py:623); step n gives
+ * t = float(n) * dt (NS_solver.py:744).  A slot is active for step n when
+ * 0 <= n < nsteps. */
+typedef struct nsk_slot {
+  float nu;
+  float dt;
+  float A;
+  float B;
+  int step;   /* base step index; kernels use step + per-launch offset */
+  int nsteps; /* total steps of this slot's rollout */
+  int pad0;
+  int pad1;
+} nsk_slot;
+
+const char *nsk_last_error(void);
+
+/* Uploads the FFT twiddle table to the current device (once per device;
+ * every other entry point calls it when needed). */
+int nsk_init(void);
+
+/* Fills *p (host memory).  Replaces the constructor arguments of
+ * SingleCoreNSSolver / Grid / Phys (NS_solver.py:715-723, 112-117). */
+int nsk_plan_init(nsk_plan *p, int N, double L, double mu0, double sigma0,
+                  double C, int batch);
+
+/* Device bytes of one spectrum buffer for the plan (all slots). */
+size_t nsk_spectrum_bytes(const nsk_plan *p);
+/* Doubles per slot of the objective partial-sum buffer for `steps` steps. */
+size_t nsk_partials_per_slot(const nsk_plan *p, int steps);
+
+/* ---- operator level (DifferentialOps, NS_solver.py:186-265) ---------- */
+/* PeriodicDifferentialOps.grad, NS_solver.py:243-247 */
+int nsk_grad(const nsk_plan *p, const float *f, float *gx, float *gy, void *stream);
+/* PeriodicDifferentialOps.laplacian, NS_solver.py:249-255 */
+int nsk_laplacian(const nsk_plan *p, const float *f, float *out, void *stream);
+/* DifferentialOps.divergence / divergence_vec, NS_solver.py:206-218 */
+int nsk_divergence(const nsk_plan *p, const float *u, float *out, void *stream);
+/* CentralAdvection.advect (scheme 0, NS_solver.py:474-477) and
+ * PeriodicDifferentialOps.upwind_advect (scheme 1, NS_solver.py:257-265) */
+int nsk_advect(const nsk_plan *p, const float *field, const float *ux,
+               const float *uy, float *out, int scheme, void *stream);
+/* GaussianForcingField.__call__, NS_solver.py:621-627,637-640: out (2,N,N) */
+int nsk_forcing(const nsk_plan *p, const float *theta2, float t, float *out,
+                void *stream);
+
+/* SpectralPoissonSolver.__call__, NS_solver.py:523-527: lap p = rhs,
+ * p_hat(0,0) = 0.  rhs and p are (N,N); spec is a spectrum buffer. */
+int nsk_poisson(const nsk_plan *p, const float *rhs, float *out, void *spec,
+                void *stream);
+/* Pressure projection of a velocity field, NS_solver.py:691-697:
+ * out = v - G L^-1 D v (the dt of the reference cancels). */
+int nsk_project(const nsk_plan *p, const float *v, float *out, void *spec,
+                void *stream);
+
+/* ChorinProjector.step / SingleCoreNSSolver.step (NS_solver.py:671-697,
+ * 727-730).  force == NULL: Gaussian forcing from slots[b].A/B at time t
+ * (or at float(step)*dt when t < 0); otherwise force is a (2,N,N) field per
+ * slot.  u and out may alias. */
+int nsk_step(const nsk_plan *p, const float *u, float *out, const float *force,
+             const nsk_slot *slots, float t, int scheme, void *spec,
+             void *stream);
+
+/* ---- rollout (SingleCoreNSSolver.rollout, NS_solver.py:732-750) ------- */
+/* Runs `steps` Chorin steps for every slot from u0 (slots[b].step must be
+ * 0 on entry and is advanced by `steps`).  J_out[b] (double, device)
+ * receives the objective.  spec0/spec1: two spectrum buffers.
+ * jpart: nsk_partials_per_slot(p, steps) doubles per slot.
+ * ckpt (nullable): stores u_n for n % ckpt_every == 0, n < steps, as
+ * ceil(steps/ckpt_every) fields of (2,N,N) per slot.
+ * use_graph != 0 captures the step sequence in a HIP graph (replayed in
+ * chunks); u_final may alias u0. */
+int nsk_rollout(const nsk_plan *p, const float *u0, float *u_final,
+                nsk_slot *slots, int steps, int scheme, int objective,
+                double *J_out, void *spec0, void *spec1, double *jpart,
+                float *ckpt, int ckpt_every, int use_graph, void *stream);
+
+/* ---- gradient (SingleCoreNSSolver.jit_gradient, NS_solver.py:725) ----- */
+/* Forward rollout with checkpoints every `ckpt_every` steps, then the
+ * hand-written discrete adjoint with segment recomputation.  Outputs per
+ * slot: J_out[b], dtheta_out[2b..2b+1] = dJ/d(A,B) (double, device),
+ * du0_out (nullable) = dJ/du0 as (2,N,N) float.  spec0..spec3: four
+ * spectrum buffers.  ckpt: ceil(steps/ckpt_every) fields of (2,N,N) per
+ * slot; seg: ckpt_every fields of (2,N,N) per slot; gpart:
+ * 2*nsk_partials_per_slot(p, steps) doubles per slot.  slots[b].step must
+ * be 0 on entry and is `steps` on return. */
+int nsk_rollout_grad(const nsk_plan *p, const float *u0, float *u_final,
+                     nsk_slot *slots, int steps, int scheme, int objective,
+                     double *J_out, double *dtheta_out, float *du0_out,
+                     void *spec0, void *spec1, void *spec2, void *spec3,
+                     double *jpart, double *gpart, float *ckpt, float *seg,
+                     int ckpt_every, void *stream);
+
+/* ---- continuous batching building blocks ------------------------------ */
+/* Enters p->batch consecutive slots (pointers already offset to the first
+ * one) at their slots[b].step: reads u0 (real, (2,N,N) per slot), writes
+ * the first predictor spectrum into spec and the step's objective partial
+ * into jpart (jcap steps per slot). */
+int nsk_admit(const nsk_plan *p, const float *u0, nsk_slot *slots, int scheme,
+              int objective, void *spec, double *jpart, int jcap, void *stream);
+/* Advances every slot by `chunk` steps, masked per slot by nsteps: a slot
+ * at step n runs the step only while n < nsteps; a slot whose step reaches
+ * nsteps writes its final state to u_final (real, (2,N,N) per slot).  The
+ * current predictor spectra are in spec_a on entry; on return they are in
+ * spec_a when chunk is even and in spec_b when it is odd.  use_graph != 0
+ * replays a cached HIP graph of the chunk.  slots[b].step advances by
+ * chunk. */
+int nsk_advance(const nsk_plan *p, nsk_slot *slots, int chunk, int scheme,
+                int objective, void *spec_a, void *spec_b, float *u_final,
+                double *jpart, int jcap, int use_graph, void *stream);
+/* out[b*ncomp + c] = sum over n < slots[b].nsteps of the partials of slot b
+ * (ncomp = 1: objective partials; 2: theta-gradient partials). */
+int nsk_finalize(const nsk_plan *p, const nsk_slot *slots, const double *part,
+                 int pcap, int ncomp, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

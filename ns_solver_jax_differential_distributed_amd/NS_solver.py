@@ -1,0 +1,611 @@
+"""Drop-in mirror of the reference's single-device solver API
+(``NS_solver.py`` of helmhlotz/ns_solver_jax_differential_distributed),
+backed by hand-written gfx950 HIP kernels (libnsk.so, through ctypes).
+
+Same class names, constructor arguments, methods and error contracts as the
+reference's single-core path:
+
+  SimConfig, Grid, Phys, Time, make_grid, load_config      (NS_solver.py:37-176)
+  DifferentialOps, PeriodicDifferentialOps                 (NS_solver.py:186-265)
+  AdvectionScheme, CentralAdvection, UpwindAdvection,
+  make_advection_scheme                                    (NS_solver.py:450-497)
+  SpectralPoissonSolver                                    (NS_solver.py:507-527)
+  GaussianForcingField                                     (NS_solver.py:606-640)
+  ChorinProjector                                          (NS_solver.py:654-697)
+  SingleCoreNSSolver (step, rollout, objective,
+                      jit_objective, jit_gradient)         (NS_solver.py:707-756)
+
+Arrays are torch tensors: fields are float32 ``(2, N, N)`` (component, x,
+y) on the GPU, like the reference's JAX arrays.  Inputs on the CPU or as
+NumPy arrays are copied to the current GPU.  The objective ``J`` is returned
+as a 0-d float64 tensor (accumulated in FP64); gradients as float64 tensors.
+
+Differences from the reference, all deliberate:
+  * N must be a power of two between 8 and 4096 for the spectral kernels
+    (``ValueError`` otherwise); the stencil operators accept any such N.
+  * ``jit_objective`` / ``jit_gradient`` are eager callables that replay
+    HIP graphs instead of XLA executables; ``jit_gradient`` runs the
+    hand-written discrete adjoint with checkpoint-every-k recomputation.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import logging
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+# ==========================
+# Logger (NS_solver.py:21-31)
+# ==========================
+def setup_logger(name: str, level: int = logging.INFO) -> logging.Logger:
+    logger = logging.getLogger(name)
+    logger.setLevel(level)
+    logger.handlers = []
+    handler = logging.StreamHandler()
+    handler.setLevel(level)
+    handler.setFormatter(logging.Formatter('[%(name)s] %(levelname)s: %(message)s'))
+    logger.addHandler(handler)
+    return logger
+
+
+logger = setup_logger(__name__)
+
+
+# ==========================
+# Configuration (NS_solver.py:37-176)
+# ==========================
+@dataclass(frozen=True)
+class SimConfig:
+    """Centralized simulation configuration (NS_solver.py:41-79)."""
+    N: int = 128
+    L: float = 1.0
+    nu: float = 0.01
+    mu0: float = 0.5
+    sigma0: float = 0.1
+    C_force: float = 1.0
+    cfl: float = 0.4
+    v_max: float = 1.0
+    t: float = 0.5
+    theta0: Optional[np.ndarray] = None
+    advection_scheme: str = "central"
+    obj_tol: float = 1e-6
+
+    def __post_init__(self):
+        if self.N <= 0:
+            raise ValueError(f"Grid resolution N must be positive, got {self.N}")
+        if self.nu <= 0:
+            raise ValueError(f"Kinematic viscosity nu must be positive, got {self.nu}")
+        if self.advection_scheme not in ("central", "upwind"):
+            raise ValueError(
+                f"Advection scheme must be 'central' or 'upwind', got {self.advection_scheme}")
+        if self.theta0 is None:
+            object.__setattr__(self, 'theta0', np.array([1.0, 0.5], dtype=np.float32))
+        else:
+            object.__setattr__(self, 'theta0', np.asarray(self.theta0, dtype=np.float32))
+
+    @property
+    def steps(self) -> int:
+        return int(np.ceil(self.t / self.dt))
+
+    @property
+    def dx(self) -> float:
+        return self.L / self.N
+
+    @property
+    def dt(self) -> float:
+        """Stable time step: float32 minimum of the CFL and diffusive limits."""
+        dt_advective = self.cfl * self.dx / self.v_max
+        dt_diffusive = (self.dx ** 2) / (4.0 * self.nu)
+        return float(np.minimum(np.float32(dt_advective), np.float32(dt_diffusive)))
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Spatial discretization of the 2D periodic domain (NS_solver.py:90-92)."""
+    N: int
+    L: float
+    dx: float
+    X: torch.Tensor
+    Y: torch.Tensor
+
+
+@dataclass(frozen=True)
+class Phys:
+    """Physical parameters of the flow (NS_solver.py:99-101)."""
+    nu: float
+    mu0: float
+    sigma0: float
+    C: float
+
+
+@dataclass(frozen=True)
+class Time:
+    """Time integration parameters (NS_solver.py:108-110)."""
+    dt: float
+    steps: int
+
+
+def make_grid(N: int, L: float = 1.0) -> Grid:
+    """Uniform 2D periodic grid on [0, L)^2, ``ij`` indexing, coordinates
+    L*(i/N) in float32 like ``jnp.linspace(..., endpoint=False)``
+    (NS_solver.py:112-117).  X, Y live on the CPU (the kernels compute the
+    coordinates themselves)."""
+    dx = L / N
+    x = torch.tensor(np.float32(L) * (np.arange(N, dtype=np.float64) / N).astype(np.float32))
+    X, Y = torch.meshgrid(x, x, indexing="ij")
+    return Grid(N, L, dx, X, Y)
+
+
+def load_config(path: Optional[str] = None) -> SimConfig:
+    """SimConfig from a JSON file with defaults for missing fields; unknown
+    keys are ignored with a warning (NS_solver.py:120-172)."""
+    defaults = {f: getattr(SimConfig(), f)
+                for f in SimConfig.__dataclass_fields__ if f != 'theta0'}
+    if path is None:
+        logger.info("No config file specified — using default SimConfig.")
+        return SimConfig()
+    if not os.path.exists(path):
+        logger.warning(f"Config file not found: '{path}' — using default SimConfig.")
+        return SimConfig()
+    with open(path, 'r') as f:
+        user = json.load(f)
+    unknown = set(user) - set(defaults) - {'theta0'}
+    if unknown:
+        logger.warning(f"Ignoring unrecognised config fields: {unknown}")
+    merged = {**defaults, **{k: v for k, v in user.items() if k in defaults}}
+    theta0 = np.asarray(user['theta0'], dtype=np.float32) if 'theta0' in user else None
+    cfg = SimConfig(**merged, theta0=theta0)
+    logger.info(f"Loaded config from '{path}': N={cfg.N}, nu={cfg.nu}, "
+                f"scheme={cfg.advection_scheme}, time={cfg.t}, steps={cfg.steps}, dt={cfg.dt:.6e}")
+    return cfg
+
+
+CONFIG = SimConfig()
+
+
+# ==========================
+# helpers
+# ==========================
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the MI355X kernels need a GPU; torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _field(a, shape_tail=None) -> torch.Tensor:
+    """float32 contiguous tensor on the current GPU."""
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.as_tensor(np.asarray(a))
+    t = t.to(device=_device(), dtype=torch.float32).contiguous()
+    if shape_tail is not None and tuple(t.shape[-len(shape_tail):]) != tuple(shape_tail):
+        raise ValueError(f"expected trailing shape {shape_tail}, got {tuple(t.shape)}")
+    return t
+
+
+def _theta_ab(theta) -> Tuple[float, float]:
+    th = np.asarray(theta.detach().cpu() if isinstance(theta, torch.Tensor) else theta,
+                    dtype=np.float32).reshape(-1)
+    if th.size < 2:
+        raise ValueError(f"theta needs at least 2 entries (A, B), got {th.size}")
+    return float(th[0]), float(th[1])
+
+
+def _check_n(N: int) -> None:
+    if N < 8 or N > 4096 or (N & (N - 1)) != 0:
+        raise ValueError(f"N={N}: the MI355X kernels support powers of two from 8 to 4096")
+
+
+_PLANS = {}
+
+
+def _plan(N: int, L: float, mu0=0.5, sigma0=0.1, C=1.0, batch=1) -> _lib.Plan:
+    key = (N, float(L), float(mu0), float(sigma0), float(C), batch)
+    p = _PLANS.get(key)
+    if p is None:
+        _check_n(N)
+        p = _lib.make_plan(N, L, mu0, sigma0, C, batch)
+        _PLANS[key] = p
+    return p
+
+
+def _spec_buffer(N: int, batch: int = 1) -> torch.Tensor:
+    """One row-spectral buffer: N*N/2 float4 per slot."""
+    return torch.empty((batch, N, N // 2, 4), dtype=torch.float32, device=_device())
+
+
+# ==========================
+# 1. Differential operators (NS_solver.py:186-265)
+# ==========================
+class DifferentialOps:
+    """Interface for grad, laplacian, divergence and upwind differencing
+    (NS_solver.py:186-234)."""
+
+    def __init__(self, dx: float):
+        self.dx = dx
+
+    def grad(self, field):
+        raise NotImplementedError
+
+    def laplacian(self, field):
+        raise NotImplementedError
+
+    def divergence(self, u):
+        gx, _ = self.grad(u[0])
+        _, gy = self.grad(u[1])
+        return gx + gy
+
+    def upwind_advect(self, field, ux, uy):
+        raise NotImplementedError
+
+    def divergence_vec(self, u):
+        return self.divergence(u)
+
+    def laplacian_and_advect_vec(self, u, advection: 'AdvectionScheme'):
+        ux, uy = u[0], u[1]
+        lap = torch.stack([self.laplacian(ux), self.laplacian(uy)], dim=0)
+        adv = torch.stack([advection.advect(ux, ux, uy, self),
+                           advection.advect(uy, ux, uy, self)], dim=0)
+        return lap, adv
+
+
+class PeriodicDifferentialOps(DifferentialOps):
+    """Periodic central / 5-point / first-order upwind stencils on the GPU
+    (NS_solver.py:237-265)."""
+
+    def _plan_for(self, N: int) -> _lib.Plan:
+        return _plan(N, self.dx * N)
+
+    def grad(self, field):
+        f = _field(field)
+        N = f.shape[-1]
+        gx = torch.empty_like(f)
+        gy = torch.empty_like(f)
+        p = self._plan_for(N)
+        _lib.check(_lib.lib().nsk_grad(ctypes.byref(p), f.data_ptr(), gx.data_ptr(), gy.data_ptr(),
+                                       _lib.stream_ptr()), "grad")
+        return gx, gy
+
+    def laplacian(self, field):
+        f = _field(field)
+        out = torch.empty_like(f)
+        p = self._plan_for(f.shape[-1])
+        _lib.check(_lib.lib().nsk_laplacian(ctypes.byref(p), f.data_ptr(), out.data_ptr(),
+                                            _lib.stream_ptr()), "laplacian")
+        return out
+
+    def divergence(self, u):
+        u = _field(u)
+        N = u.shape[-1]
+        out = torch.empty((N, N), dtype=torch.float32, device=u.device)
+        p = self._plan_for(N)
+        _lib.check(_lib.lib().nsk_divergence(ctypes.byref(p), u.data_ptr(), out.data_ptr(),
+                                             _lib.stream_ptr()), "divergence")
+        return out
+
+    def advect(self, field, ux, uy, scheme: int):
+        f, a, b = _field(field), _field(ux), _field(uy)
+        out = torch.empty_like(f)
+        p = self._plan_for(f.shape[-1])
+        _lib.check(_lib.lib().nsk_advect(ctypes.byref(p), f.data_ptr(), a.data_ptr(), b.data_ptr(),
+                                         out.data_ptr(), scheme, _lib.stream_ptr()), "advect")
+        return out
+
+    def upwind_advect(self, field, ux, uy):
+        return self.advect(field, ux, uy, _lib.SCHEME["upwind"])
+
+
+# ==========================
+# 2. Advection schemes (NS_solver.py:450-497)
+# ==========================
+class AdvectionScheme:
+    """Interface for -(u.grad)u discretizations (NS_solver.py:450-465)."""
+    name = None
+
+    def advect(self, field, ux, uy, ops: DifferentialOps):
+        raise NotImplementedError
+
+
+class CentralAdvection(AdvectionScheme):
+    """-(ux d_x phi + uy d_y phi) with central differences (NS_solver.py:468-477)."""
+    name = "central"
+
+    def advect(self, field, ux, uy, ops: DifferentialOps):
+        if isinstance(ops, PeriodicDifferentialOps):
+            return ops.advect(field, ux, uy, _lib.SCHEME["central"])
+        fx, fy = ops.grad(field)
+        return -(ux * fx + uy * fy)
+
+
+class UpwindAdvection(AdvectionScheme):
+    """First-order upwind (NS_solver.py:480-488)."""
+    name = "upwind"
+
+    def advect(self, field, ux, uy, ops: DifferentialOps):
+        return ops.upwind_advect(field, ux, uy)
+
+
+def make_advection_scheme(name: str) -> AdvectionScheme:
+    """Factory (NS_solver.py:491-497)."""
+    if name == "central":
+        return CentralAdvection()
+    elif name == "upwind":
+        return UpwindAdvection()
+    raise ValueError(f"Unknown advection scheme: '{name}'. Choose 'central' or 'upwind'.")
+
+
+# ==========================
+# 3. Spectral Poisson solver (NS_solver.py:507-527)
+# ==========================
+class SpectralPoissonSolver:
+    """Solves lap p = rhs spectrally with p_hat(0,0) = 0 (NS_solver.py:507-527:
+    p_hat = -fft2(rhs)/|k|^2 - the sign convention of the reference's code,
+    not of its docstring)."""
+
+    def __init__(self, Nx: int, Ny: int, dx: float):
+        if Nx != Ny:
+            raise ValueError(f"square grids only, got {Nx}x{Ny}")
+        _check_n(Nx)
+        self.N = Nx
+        self.dx = dx
+        self._spec = None
+
+    @property
+    def _inv_k2(self) -> torch.Tensor:
+        """1/|k|^2 with the k=0 entry 0 (NS_solver.py:518-521), float32, CPU."""
+        k = torch.tensor(np.fft.fftfreq(self.N, d=self.dx) * 2 * np.pi, dtype=torch.float32)
+        KX, KY = torch.meshgrid(k, k, indexing="ij")
+        k2 = KX ** 2 + KY ** 2
+        return torch.where(k2 == 0, torch.zeros_like(k2), 1.0 / torch.where(k2 == 0, torch.ones_like(k2), k2))
+
+    def _buffer(self):
+        if self._spec is None or self._spec.device != _device():
+            self._spec = _spec_buffer(self.N)
+        return self._spec
+
+    def __call__(self, rhs):
+        r = _field(rhs, (self.N, self.N))
+        out = torch.empty_like(r)
+        p = _plan(self.N, self.dx * self.N)
+        _lib.check(_lib.lib().nsk_poisson(ctypes.byref(p), r.data_ptr(), out.data_ptr(),
+                                          self._buffer().data_ptr(), _lib.stream_ptr()), "poisson")
+        return out
+
+    def project(self, v):
+        """v - G L^-1 D v for a (2,N,N) field (the pressure projection of
+        NS_solver.py:691-697 with dt cancelled)."""
+        u = _field(v, (2, self.N, self.N))
+        out = torch.empty_like(u)
+        p = _plan(self.N, self.dx * self.N)
+        _lib.check(_lib.lib().nsk_project(ctypes.byref(p), u.data_ptr(), out.data_ptr(),
+                                          self._buffer().data_ptr(), _lib.stream_ptr()), "project")
+        return out
+
+
+# ==========================
+# 4. Forcing field (NS_solver.py:606-640)
+# ==========================
+class GaussianForcingField:
+    """f(x,t;theta) = C exp(-d(x,mu(t))^2 / 2 sigma(t)^2) e_x with
+    mu = mu0 + sin(2 pi A t), sigma = sigma0 + cos^2(2 pi B t) and the
+    period-1 wrap d = (x-mu) - round(x-mu) (NS_solver.py:606-640)."""
+
+    def __init__(self, grid: Grid, phys: Phys):
+        self.grid = grid
+        self.phys = phys
+
+    def _x_component(self, theta, t, x):
+        """Scalar profile at coordinates x (float32 torch arithmetic on x's
+        device; NS_solver.py:621-627)."""
+        A, B = _theta_ab(theta)
+        x = torch.as_tensor(x, dtype=torch.float32)
+        t32 = torch.tensor(np.float32(t), device=x.device)
+        two_pi = torch.tensor(np.float32(2.0 * np.pi), device=x.device)
+        mu_t = np.float32(self.phys.mu0) + torch.sin(two_pi * np.float32(A) * t32)
+        sig_t = np.float32(self.phys.sigma0) + torch.cos(two_pi * np.float32(B) * t32) ** 2
+        d = (x - mu_t) - torch.round(x - mu_t)
+        return np.float32(self.phys.C) * torch.exp(-(d * d) / (2.0 * sig_t * sig_t))
+
+    def __call__(self, theta, t):
+        """(2, N, N) forcing at time t, computed on the GPU."""
+        A, B = _theta_ab(theta)
+        N = self.grid.N
+        out = torch.empty((2, N, N), dtype=torch.float32, device=_device())
+        th = torch.tensor([A, B], dtype=torch.float32, device=out.device)
+        p = _plan(N, self.grid.L, self.phys.mu0, self.phys.sigma0, self.phys.C)
+        _lib.check(_lib.lib().nsk_forcing(ctypes.byref(p), th.data_ptr(), float(np.float32(t)),
+                                          out.data_ptr(), _lib.stream_ptr()), "forcing")
+        return out
+
+
+# ==========================
+# 5. Chorin projector (NS_solver.py:654-697)
+# ==========================
+class ChorinProjector:
+    """One explicit-Euler + pressure-projection step (NS_solver.py:654-697).
+
+    With ``PeriodicDifferentialOps`` and ``SpectralPoissonSolver`` the whole
+    step runs as the fused HIP pipeline (row pass, column pass, row pass);
+    other operator objects are composed op by op like the reference."""
+
+    def __init__(self, ops: DifferentialOps, poisson: SpectralPoissonSolver,
+                 advection: AdvectionScheme, nu: float, dt: float):
+        self.ops = ops
+        self.poisson = poisson
+        self.advection = advection
+        self.nu = nu
+        self.dt = dt
+        self._slot = None
+        self._spec = None
+
+    def _fused(self) -> bool:
+        return (type(self.ops) is PeriodicDifferentialOps
+                and type(self.poisson) is SpectralPoissonSolver
+                and getattr(self.advection, "name", None) in _lib.SCHEME)
+
+    def _slots(self, A=0.0, B=0.0):
+        return _lib.make_slots(self.nu, self.dt, A, B)
+
+    def _run(self, u, f, theta, t):
+        u = _field(u)
+        N = u.shape[-1]
+        if self._spec is None or self._spec.shape[1] != N:
+            self._spec = _spec_buffer(N)
+        out = torch.empty_like(u)
+        A, B = (0.0, 0.0) if theta is None else _theta_ab(theta)
+        slots = self._slots(A, B)
+        force = None if f is None else _field(f, (2, N, N))
+        p = _plan(N, self.ops.dx * N, *getattr(self, "_phys", (0.5, 0.1, 1.0)))
+        _lib.check(_lib.lib().nsk_step(
+            ctypes.byref(p), u.data_ptr(), out.data_ptr(),
+            None if force is None else force.data_ptr(), slots.data_ptr(),
+            float(np.float32(t)) if t is not None else -1.0,
+            _lib.SCHEME[self.advection.name], self._spec.data_ptr(), _lib.stream_ptr()), "step")
+        return out
+
+    def step(self, u, f):
+        """u^{n+1} from u^n and the body force f, both (2, N, N)."""
+        if self._fused():
+            return self._run(u, f, None, 0.0)
+        ux, uy = u[0], u[1]
+        ops, dt, nu = self.ops, self.dt, self.nu
+        lap_u, adv_u = ops.laplacian_and_advect_vec(u, self.advection)
+        u_star = torch.stack([ux + dt * (nu * lap_u[0] + adv_u[0] + f[0]),
+                              uy + dt * (nu * lap_u[1] + adv_u[1] + f[1])], dim=0)
+        div_star = ops.divergence_vec(u_star)
+        p = self.poisson(div_star / dt)
+        px, py = ops.grad(p)
+        return torch.stack([u_star[0] - dt * px, u_star[1] - dt * py], dim=0)
+
+
+# ==========================
+# 6. Single-device solver (NS_solver.py:707-756)
+# ==========================
+class SingleCoreNSSolver:
+    """Single-device NS solver: rollout, objective and gradient on the GPU
+    (NS_solver.py:707-756).
+
+    Extra keyword arguments (not in the reference):
+      objective: "work" (reference J, NS_solver.py:746) or "mean_ke"
+      ckpt_every: checkpoint interval of the adjoint's recomputation
+      use_graph: replay the step sequence through HIP graphs
+    """
+
+    def __init__(self, grid: Grid, phys: Phys, time_cfg: Time, advection_scheme: str = "central",
+                 objective: str = "work", ckpt_every: int = 50, use_graph: bool = True):
+        self.grid = grid
+        self.time = time_cfg
+        self.phys = phys
+        ops = PeriodicDifferentialOps(grid.dx)
+        poisson = SpectralPoissonSolver(grid.N, grid.N, grid.dx)
+        advection = make_advection_scheme(advection_scheme)
+        if objective not in _lib.OBJECTIVE:
+            raise ValueError(f"objective must be 'work' or 'mean_ke', got {objective!r}")
+        if int(ckpt_every) < 1:
+            raise ValueError("ckpt_every must be >= 1")
+        self.forcing = GaussianForcingField(grid, phys)
+        self.projector = ChorinProjector(ops, poisson, advection, phys.nu, time_cfg.dt)
+        self.projector._phys = (phys.mu0, phys.sigma0, phys.C)
+        self.scheme = advection_scheme
+        self.objective_kind = objective
+        self.ckpt_every = int(ckpt_every)
+        self.use_graph = bool(use_graph)
+        self._ws = {}
+        self.jit_objective = self.objective
+        self.jit_gradient = self.gradient
+
+    # -- workspace (kept across calls so graph replays see stable pointers)
+    def _buf(self, name, shape, dtype=torch.float32):
+        t = self._ws.get(name)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype or t.device != _device():
+            t = torch.empty(shape, dtype=dtype, device=_device())
+            self._ws[name] = t
+        return t
+
+    def _plan(self) -> _lib.Plan:
+        return _plan(self.grid.N, self.grid.L, self.phys.mu0, self.phys.sigma0, self.phys.C)
+
+    def step(self, u, theta, t):
+        """Single time step with the Gaussian forcing at time t (NS_solver.py:727-730)."""
+        return self.projector._run(u, None, theta, t)
+
+    def rollout(self, u0, theta):
+        """Integrate self.time.steps steps from u0 accumulating
+        J = sum_i dt * sum(u_i . f_i) * dx^2 before each step
+        (NS_solver.py:732-750).  Returns (J, u_final)."""
+        N, steps = self.grid.N, int(self.time.steps)
+        u0 = _field(u0, (2, N, N))
+        A, B = _theta_ab(theta)
+        p = self._plan()
+        slots = _lib.make_slots(self.phys.nu, self.time.dt, A, B, 0, steps)
+        s0 = self._buf("spec0", (1, N, N // 2, 4))
+        s1 = self._buf("spec1", (1, N, N // 2, 4))
+        nb = max(1, int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), max(steps, 1))))
+        jpart = self._buf("jpart", (nb,), torch.float64)
+        J = torch.empty((1,), dtype=torch.float64, device=u0.device)
+        u_final = torch.empty_like(u0)
+        _lib.check(_lib.lib().nsk_rollout(
+            ctypes.byref(p), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
+            _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective_kind], J.data_ptr(),
+            s0.data_ptr(), s1.data_ptr(), jpart.data_ptr(), None, 1, int(self.use_graph),
+            _lib.stream_ptr()), "rollout")
+        return J[0], u_final
+
+    def rollout_and_grad(self, u0, theta):
+        """(J, u_final, dJ/dtheta, dJ/du0) by the hand-written discrete
+        adjoint with checkpoints every ``ckpt_every`` steps."""
+        N, steps = self.grid.N, int(self.time.steps)
+        u0 = _field(u0, (2, N, N))
+        th = np.asarray(theta.detach().cpu() if isinstance(theta, torch.Tensor) else theta,
+                        dtype=np.float64).reshape(-1)
+        A, B = _theta_ab(th)
+        if steps < 1:
+            z = torch.zeros((), dtype=torch.float64, device=u0.device)
+            return z, u0.clone(), torch.zeros(len(th), dtype=torch.float64, device=u0.device), \
+                torch.zeros_like(u0)
+        k = min(self.ckpt_every, steps)
+        ncheck = (steps + k - 1) // k
+        p = self._plan()
+        slots = _lib.make_slots(self.phys.nu, self.time.dt, A, B, 0, steps)
+        specs = [self._buf(f"spec{i}", (1, N, N // 2, 4)) for i in range(4)]
+        nb = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), steps))
+        jpart = self._buf("jpart", (nb,), torch.float64)
+        gpart = self._buf("gpart", (2 * nb,), torch.float64)
+        ckpt = self._buf("ckpt", (ncheck, 2, N, N))
+        seg = self._buf("seg", (k, 2, N, N))
+        J = torch.empty((1,), dtype=torch.float64, device=u0.device)
+        dth = torch.empty((2,), dtype=torch.float64, device=u0.device)
+        du0 = torch.empty_like(u0)
+        u_final = torch.empty_like(u0)
+        _lib.check(_lib.lib().nsk_rollout_grad(
+            ctypes.byref(p), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
+            _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective_kind], J.data_ptr(),
+            dth.data_ptr(), du0.data_ptr(), *[s.data_ptr() for s in specs], jpart.data_ptr(),
+            gpart.data_ptr(), ckpt.data_ptr(), seg.data_ptr(), k, _lib.stream_ptr()), "rollout_grad")
+        g = torch.zeros(len(th), dtype=torch.float64, device=u0.device)
+        g[:2] = dth
+        return J[0], u_final, g, du0
+
+    def objective(self, theta):
+        """J(theta) from the zero initial condition (NS_solver.py:752-756)."""
+        u0 = torch.zeros((2, self.grid.N, self.grid.N), dtype=torch.float32, device=_device())
+        J, _ = self.rollout(u0, theta)
+        return J
+
+    def gradient(self, theta):
+        """dJ/dtheta from the zero initial condition, same length as theta
+        (entries past the second are exactly 0: the forcing reads only
+        theta[0] and theta[1])."""
+        u0 = torch.zeros((2, self.grid.N, self.grid.N), dtype=torch.float32, device=_device())
+        _, _, g, _ = self.rollout_and_grad(u0, theta)
+        return g

@@ -1,0 +1,153 @@
+"""ctypes binding of libnsk.so (the C ABI declared in include/nsk.h).
+
+PyTorch tensors are the only device containers: every device pointer passed
+to the library is a ``tensor.data_ptr()`` and work is enqueued on
+``torch.cuda.current_stream()``.  There is no fallback: if the library or a
+GPU is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnsk.so")
+
+NSK_OK = 0
+NSK_E_INVAL = -1
+NSK_E_UNSUPPORTED_N = -2
+NSK_E_HIP = -3
+NSK_E_WORKSPACE = -4
+
+SCHEME = {"central": 0, "upwind": 1}
+OBJECTIVE = {"work": 0, "mean_ke": 1}
+
+# Every entry point of include/nsk.h; tests check the library exports each.
+EXPORTED = (
+    "nsk_last_error", "nsk_init", "nsk_plan_init", "nsk_spectrum_bytes",
+    "nsk_partials_per_slot", "nsk_grad", "nsk_laplacian", "nsk_divergence",
+    "nsk_advect", "nsk_forcing", "nsk_poisson", "nsk_project", "nsk_step",
+    "nsk_rollout", "nsk_rollout_grad", "nsk_admit", "nsk_advance",
+    "nsk_finalize",
+)
+
+
+class Plan(ctypes.Structure):
+    """Mirror of ``nsk_plan``."""
+    _fields_ = [
+        ("N", ctypes.c_int), ("batch", ctypes.c_int),
+        ("L", ctypes.c_double), ("dx", ctypes.c_double),
+        ("Lf", ctypes.c_float), ("mu0", ctypes.c_float),
+        ("sigma0", ctypes.c_float), ("C", ctypes.c_float),
+        ("inv_dx", ctypes.c_float), ("inv_2dx", ctypes.c_float),
+        ("inv_dx2", ctypes.c_float), ("kscale", ctypes.c_float),
+    ]
+
+
+SLOT_WORDS = 8  # nsk_slot: nu, dt, A, B (float) | step, nsteps, pad, pad (int)
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_P = ctypes.POINTER(Plan)
+
+_SIGS = {
+    "nsk_last_error": ([], ctypes.c_char_p),
+    "nsk_init": ([], _i),
+    "nsk_plan_init": ([_P, _i, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                       ctypes.c_double, _i], _i),
+    "nsk_spectrum_bytes": ([_P], ctypes.c_size_t),
+    "nsk_partials_per_slot": ([_P, _i], ctypes.c_size_t),
+    "nsk_grad": ([_P, _vp, _vp, _vp, _vp], _i),
+    "nsk_laplacian": ([_P, _vp, _vp, _vp], _i),
+    "nsk_divergence": ([_P, _vp, _vp, _vp], _i),
+    "nsk_advect": ([_P, _vp, _vp, _vp, _vp, _i, _vp], _i),
+    "nsk_forcing": ([_P, _vp, _f, _vp, _vp], _i),
+    "nsk_poisson": ([_P, _vp, _vp, _vp, _vp], _i),
+    "nsk_project": ([_P, _vp, _vp, _vp, _vp], _i),
+    "nsk_step": ([_P, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp], _i),
+    "nsk_rollout": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
+    "nsk_rollout_grad": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                          _vp, _vp, _vp, _vp, _i, _vp], _i),
+    "nsk_admit": ([_P, _vp, _vp, _i, _i, _vp, _vp, _i, _vp], _i),
+    "nsk_advance": ([_P, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
+    "nsk_finalize": ([_P, _vp, _vp, _i, _i, _vp, _vp], _i),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libnsk.so (no device work).  Raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise RuntimeError(
+                    f"{path} not found: build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'`")
+            lib = ctypes.CDLL(path)
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = res
+            _lib = lib
+    return _lib
+
+
+def lib() -> ctypes.CDLL:
+    """The library, for calls that enqueue GPU work."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("the MI355X kernels need a GPU; torch.cuda.is_available() is False")
+    return load_library()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == NSK_OK:
+        return
+    msg = load_library().nsk_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc in (NSK_E_INVAL, NSK_E_UNSUPPORTED_N):
+        raise ValueError(text)
+    raise RuntimeError(text)
+
+
+def make_plan(N: int, L: float, mu0: float = 0.5, sigma0: float = 0.1, C: float = 1.0,
+              batch: int = 1) -> Plan:
+    p = Plan()
+    check(load_library().nsk_plan_init(ctypes.byref(p), int(N), float(L), float(mu0),
+                                       float(sigma0), float(C), int(batch)), "nsk_plan_init")
+    return p
+
+
+def stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a tensor on the GPU")
+    return t.data_ptr()
+
+
+def make_slots(nu, dt, A, B, step=0, nsteps=0, batch: int = 1, device=None) -> torch.Tensor:
+    """Device table of ``nsk_slot`` records, one per slot (int32 words; the
+    float fields are reinterpreted in place)."""
+    import numpy as np
+    rec = np.zeros((batch, SLOT_WORDS), dtype=np.int32)
+    f = rec.view(np.float32)
+    f[:, 0] = np.broadcast_to(np.asarray(nu, dtype=np.float32), (batch,))
+    f[:, 1] = np.broadcast_to(np.asarray(dt, dtype=np.float32), (batch,))
+    f[:, 2] = np.broadcast_to(np.asarray(A, dtype=np.float32), (batch,))
+    f[:, 3] = np.broadcast_to(np.asarray(B, dtype=np.float32), (batch,))
+    rec[:, 4] = np.broadcast_to(np.asarray(step, dtype=np.int32), (batch,))
+    rec[:, 5] = np.broadcast_to(np.asarray(nsteps, dtype=np.int32), (batch,))
+    dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+    return torch.from_numpy(rec).to(dev)

@@ -1,0 +1,742 @@
+// nsk_api.hip - the extern "C" entry points of libnsk.so: argument checks,
+// dispatch to the per-N row/column kernels, the operator-level stencil
+// kernels, the rollout and adjoint drivers and the HIP-graph cache.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "nsk_internal.h"
+
+using namespace nsk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(NSK_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+bool supported_n(int N) { return N >= 8 && N <= 4096 && (N & (N - 1)) == 0; }
+
+hipError_t launch_row(int N, int kind, int scheme, const RowArgs& a, int batch, hipStream_t s) {
+  switch (N) {
+#define NSK_CASE(n) \
+  case n: return launch_row_##n(kind, scheme, a, batch, s);
+    NSK_CASE(8) NSK_CASE(16) NSK_CASE(32) NSK_CASE(64) NSK_CASE(128)
+    NSK_CASE(256) NSK_CASE(512) NSK_CASE(1024) NSK_CASE(2048) NSK_CASE(4096)
+#undef NSK_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_col(int N, int mode, const ColArgs& a, int batch, hipStream_t s) {
+  switch (N) {
+#define NSK_CASE(n) \
+  case n: return launch_col_##n(mode, a, batch, s);
+    NSK_CASE(8) NSK_CASE(16) NSK_CASE(32) NSK_CASE(64) NSK_CASE(128)
+    NSK_CASE(256) NSK_CASE(512) NSK_CASE(1024) NSK_CASE(2048) NSK_CASE(4096)
+#undef NSK_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+std::mutex g_init_mu;
+std::vector<bool> g_init_done;
+
+int ensure_init() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if ((int)g_init_done.size() > dev && g_init_done[dev]) return NSK_OK;
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_TRY(launch_init_tw_8(s));
+  HIP_TRY(launch_init_tw_16(s));
+  HIP_TRY(launch_init_tw_32(s));
+  HIP_TRY(launch_init_tw_64(s));
+  HIP_TRY(launch_init_tw_128(s));
+  HIP_TRY(launch_init_tw_256(s));
+  HIP_TRY(launch_init_tw_512(s));
+  HIP_TRY(launch_init_tw_1024(s));
+  HIP_TRY(launch_init_tw_2048(s));
+  HIP_TRY(launch_init_tw_4096(s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipStreamDestroy(s));
+  if ((int)g_init_done.size() <= dev) g_init_done.resize(dev + 1, false);
+  g_init_done[dev] = true;
+  return NSK_OK;
+}
+
+int check_plan(const nsk_plan* p) {
+  if (!p) return fail(NSK_E_INVAL, "plan is NULL");
+  if (!supported_n(p->N))
+    return fail(NSK_E_UNSUPPORTED_N, "N=%d: the HIP path supports powers of two 8..4096", p->N);
+  if (p->batch < 1) return fail(NSK_E_INVAL, "batch must be >= 1, got %d", p->batch);
+  return NSK_OK;
+}
+
+#define CHECK_PLAN(p)                     \
+  do {                                    \
+    int rc_ = check_plan(p);              \
+    if (rc_) return rc_;                  \
+    rc_ = ensure_init();                  \
+    if (rc_) return rc_;                  \
+  } while (0)
+
+RowArgs row_args(const nsk_plan* p, const nsk_slot* slots) {
+  RowArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.plan = *p;
+  a.slots = slots;
+  a.src_nc = 2;
+  a.dst_nc = 2;
+  a.t_explicit = -1.0f;
+  a.st_every = 1;
+  return a;
+}
+
+ColArgs col_args(const nsk_plan* p, void* spec, const nsk_slot* slots, int off, int req) {
+  ColArgs c;
+  std::memset(&c, 0, sizeof c);
+  c.plan = *p;
+  c.spec = (float4*)spec;
+  c.slots = slots;
+  c.step_off = off;
+  c.require = req;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// small device kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float op_forcing_x(const float* th, float t, float x, const nsk_plan& p) {
+  const float two_pi = 6.28318530717958647692f;
+  const float mu = p.mu0 + sinf(two_pi * th[0] * t);
+  const float cb = cosf(two_pi * th[1] * t);
+  const float sig = p.sigma0 + cb * cb;
+  const float xm = x - mu;
+  const float d = xm - rintf(xm);
+  return p.C * expf(-(d * d) / (2.0f * sig * sig));
+}
+
+// Operator-level stencils (NS_solver.py:206-265, 474-477, 621-640), one
+// thread per cell, blockIdx.y = slot.
+__global__ void k_stencil(StencilArgs a) {
+  const int N = a.plan.N;
+  const size_t NN = (size_t)N * N;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NN) return;
+  const int x = (int)(i / N), y = (int)(i % N);
+  const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+  const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
+  const long long bi = (long long)blockIdx.y * a.in_bstride;
+  const long long bo = (long long)blockIdx.y * a.out_bstride;
+  const nsk_plan& p = a.plan;
+#define AT(ptr, xx, yy) (ptr)[bi + (size_t)(xx) * N + (yy)]
+  switch (a.op) {
+    case 0: {  // grad
+      const float* f = a.a;
+      a.out0[bo + i] = (AT(f, xp, y) - AT(f, xm, y)) * p.inv_2dx;
+      a.out1[bo + i] = (AT(f, x, yp) - AT(f, x, ym)) * p.inv_2dx;
+      break;
+    }
+    case 1: {  // laplacian
+      const float* f = a.a;
+      a.out0[bo + i] = (AT(f, xm, y) + AT(f, xp, y) + AT(f, x, yp) + AT(f, x, ym) - 4.0f * AT(f, x, y)) * p.inv_dx2;
+      break;
+    }
+    case 2: {  // divergence of (2,N,N)
+      const float* ux = a.a;
+      const float* uy = a.a + NN;
+      const float gx = (AT(ux, xp, y) - AT(ux, xm, y)) * p.inv_2dx;
+      const float gy = (AT(uy, x, yp) - AT(uy, x, ym)) * p.inv_2dx;
+      a.out0[bo + i] = gx + gy;
+      break;
+    }
+    case 3: {  // central advection of field a.a by (a.b, a.c)
+      const float* f = a.a;
+      const float fx = (AT(f, xp, y) - AT(f, xm, y)) * p.inv_2dx;
+      const float fy = (AT(f, x, yp) - AT(f, x, ym)) * p.inv_2dx;
+      a.out0[bo + i] = -(AT(a.b, x, y) * fx + AT(a.c, x, y) * fy);
+      break;
+    }
+    case 4: {  // upwind advection
+      const float* f = a.a;
+      const float c = AT(f, x, y);
+      const float ux = AT(a.b, x, y), uy = AT(a.c, x, y);
+      const float dfx = (ux >= 0.0f ? c - AT(f, xm, y) : AT(f, xp, y) - c) * p.inv_dx;
+      const float dfy = (uy >= 0.0f ? c - AT(f, x, ym) : AT(f, x, yp) - c) * p.inv_dx;
+      a.out0[bo + i] = -(ux * dfx + uy * dfy);
+      break;
+    }
+    case 5: {  // forcing field (2,N,N)
+      const float xc = p.Lf * ((float)x / (float)N);
+      a.out0[bo + i] = op_forcing_x(a.theta, a.t, xc, p);
+      a.out0[bo + NN + i] = 0.0f;
+      break;
+    }
+  }
+#undef AT
+}
+
+// out[b*ncomp + c] = sum_{n < nsteps_b, band} part[b][n][band][c], in a
+// fixed order (thread-strided partial sums, then a fixed tree).
+__global__ void k_finalize(const double* part, int pcap, int nb, int ncomp,
+                           const nsk_slot* slots, double* out) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  int nsteps = slots[b].nsteps;
+  if (nsteps > pcap) nsteps = pcap;
+  const size_t total = (size_t)nsteps * nb;
+  const double* base = part + (size_t)b * pcap * nb * ncomp;
+  double s = 0.0;
+  for (size_t k = tid; k < total; k += 256) s += base[k * ncomp + c];
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) out[(size_t)b * ncomp + c] = red[0];
+}
+
+__global__ void k_set_step(nsk_slot* s, int batch, int value) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) s[b].step = value;
+}
+
+__global__ void k_add_step(nsk_slot* s, int batch, int delta) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) s[b].step += delta;
+}
+
+// Fill kernel (used instead of hipMemsetAsync so it may sit in a graph).
+__global__ void k_zero(float4* p, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_zero_d(double* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+hipError_t set_step(nsk_slot* s, int batch, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_step, dim3((batch + 255) / 256), dim3(256), 0, st, s, batch, v);
+  return hipGetLastError();
+}
+hipError_t add_step(nsk_slot* s, int batch, int d, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_step, dim3((batch + 255) / 256), dim3(256), 0, st, s, batch, d);
+  return hipGetLastError();
+}
+hipError_t zero_spec(const nsk_plan* p, void* spec, hipStream_t st) {
+  const size_t n4 = (size_t)p->batch * p->N * (p->N / 2);
+  hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, (float4*)spec, n4);
+  return hipGetLastError();
+}
+hipError_t finalize(const nsk_plan* p, const nsk_slot* slots, const double* part, int pcap,
+                    int ncomp, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(p->batch, ncomp), dim3(256), 0, st, part, pcap,
+                     row_bands(p->N), ncomp, slots, out);
+  return hipGetLastError();
+}
+
+int stencil(const nsk_plan* p, int op, const float* a0, const float* b0, const float* c0,
+            float* o0, float* o1, long long in_bs, long long out_bs, const float* theta,
+            float t, hipStream_t st) {
+  StencilArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.plan = *p;
+  a.op = op;
+  a.a = a0;
+  a.b = b0;
+  a.c = c0;
+  a.out0 = o0;
+  a.out1 = o1;
+  a.in_bstride = in_bs;
+  a.out_bstride = out_bs;
+  a.theta = theta;
+  a.t = t;
+  const size_t NN = (size_t)p->N * p->N;
+  hipLaunchKernelGGL(k_stencil, dim3((unsigned)((NN + 255) / 256), p->batch), dim3(256), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  return NSK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one chunk of rollout steps: for o in [0, chunk):
+//   column pass (projection of the predictor of step n = step+o)
+//   row pass    (step n+1: predictor of u_{n+1}, or the final real state)
+// then step += chunk.  Ping-pongs between spec_a and spec_b.
+// ---------------------------------------------------------------------------
+hipError_t enqueue_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective,
+                         void* spec_a, void* spec_b, float* u_final, double* jpart, int jcap,
+                         float* ckpt, int ckpt_every, int ckpt_cap, hipStream_t st) {
+  void* cur = spec_a;
+  void* nxt = spec_b;
+  const long long NN2 = 2LL * p->N * p->N;
+  for (int o = 0; o < chunk; ++o) {
+    ColArgs c = col_args(p, cur, slots, o, REQ_ACTIVE);
+    hipError_t e = launch_col(p->N, COL_PROJECT, c, p->batch, st);
+    if (e != hipSuccess) return e;
+    RowArgs r = row_args(p, slots);
+    r.src_spec = (const float4*)cur;
+    r.dst_spec = (float4*)nxt;
+    r.dst_real = u_final;
+    r.dst_bstride = NN2;
+    r.step_off = o + 1;
+    r.require = REQ_ROLL;
+    r.part = jpart;
+    r.pcap = jcap;
+    r.objective = objective;
+    if (ckpt) {
+      r.ustore = ckpt;
+      r.st_bstride = (long long)ckpt_cap * NN2;
+      r.st_base = 0;
+      r.st_every = ckpt_every;
+      r.st_cap = ckpt_cap;
+    }
+    e = launch_row(p->N, RK_STEP_ROLL, scheme, r, p->batch, st);
+    if (e != hipSuccess) return e;
+    std::swap(cur, nxt);
+  }
+  return add_step(slots, p->batch, chunk, st);
+}
+
+struct GraphKey {
+  int dev, N, batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap;
+  const void *slots, *a, *b, *uf, *jp, *ck;
+  double L, mu0, sigma0, C;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(dev, N, batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap, slots, a, b,
+                    uf, jp, ck, L, mu0, sigma0, C) <
+           std::tie(o.dev, o.N, o.batch, o.chunk, o.scheme, o.objective, o.jcap, o.ckpt_every,
+                    o.ckpt_cap, o.slots, o.a, o.b, o.uf, o.jp, o.ck, o.L, o.mu0, o.sigma0, o.C);
+  }
+};
+std::mutex g_graph_mu;
+std::map<GraphKey, hipGraphExec_t> g_graphs;
+std::vector<hipStream_t> g_capture_streams;
+constexpr size_t kMaxGraphs = 64;
+
+// Replays (capturing on first use) the graph of one chunk.
+int run_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective,
+              void* spec_a, void* spec_b, float* u_final, double* jpart, int jcap, float* ckpt,
+              int ckpt_every, int ckpt_cap, bool use_graph, hipStream_t st) {
+  if (!use_graph) {
+    HIP_TRY(enqueue_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart, jcap,
+                          ckpt, ckpt_every, ckpt_cap, st));
+    return NSK_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  GraphKey k{dev, p->N, p->batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap,
+             slots, spec_a, spec_b, u_final, jpart, ckpt, p->L, (double)p->mu0,
+             (double)p->sigma0, (double)p->C};
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(k);
+    if (it != g_graphs.end()) exec = it->second;
+  }
+  if (!exec) {
+    // Capture on a private stream (the caller's may be the NULL stream,
+    // which cannot be captured); capturing records work without running it.
+    hipStream_t cs = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_graph_mu);
+      if ((int)g_capture_streams.size() <= dev) g_capture_streams.resize(dev + 1, nullptr);
+      if (!g_capture_streams[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_capture_streams[dev], hipStreamNonBlocking));
+      cs = g_capture_streams[dev];
+    }
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart,
+                                 jcap, ckpt, ckpt_every, ckpt_cap, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (e != hipSuccess) return fail(NSK_E_HIP, "capture of rollout chunk failed: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(NSK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphDestroy(graph));
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    if (g_graphs.size() >= kMaxGraphs) {
+      for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+      g_graphs.clear();
+    }
+    g_graphs[k] = exec;
+  }
+  HIP_TRY(hipGraphLaunch(exec, st));
+  return NSK_OK;
+}
+
+// Steps `steps` from the predictor spectrum in *cur; keeps the ping-pong
+// parity in (*cur, *nxt).  Even-sized chunks go through the graph cache.
+int run_steps(const nsk_plan* p, nsk_slot* slots, int steps, int scheme, int objective,
+              void** cur, void** nxt, float* u_final, double* jpart, int jcap, float* ckpt,
+              int ckpt_every, int ckpt_cap, bool use_graph, hipStream_t st) {
+  constexpr int kChunk = 32;
+  int left = steps;
+  while (left > 0) {
+    int c = left >= kChunk ? kChunk : left;
+    if (use_graph && c >= 2 && (c & 1)) c -= 1;
+    const bool g = use_graph && c == kChunk;
+    int rc = run_chunk(p, slots, c, scheme, objective, *cur, *nxt, u_final, jpart, jcap, ckpt,
+                       ckpt_every, ckpt_cap, g, st);
+    if (rc) return rc;
+    if (c & 1) std::swap(*cur, *nxt);
+    left -= c;
+  }
+  return NSK_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* nsk_last_error(void) { return g_err.c_str(); }
+
+int nsk_init(void) { return ensure_init(); }
+
+int nsk_plan_init(nsk_plan* p, int N, double L, double mu0, double sigma0, double C, int batch) {
+  if (!p) return fail(NSK_E_INVAL, "plan is NULL");
+  if (!supported_n(N))
+    return fail(NSK_E_UNSUPPORTED_N, "N=%d: the HIP path supports powers of two 8..4096", N);
+  if (!(L > 0.0)) return fail(NSK_E_INVAL, "L must be positive, got %g", L);
+  if (batch < 1) return fail(NSK_E_INVAL, "batch must be >= 1, got %d", batch);
+  std::memset(p, 0, sizeof *p);
+  p->N = N;
+  p->batch = batch;
+  p->L = L;
+  p->dx = L / N;
+  p->Lf = (float)L;
+  p->mu0 = (float)mu0;
+  p->sigma0 = (float)sigma0;
+  p->C = (float)C;
+  p->inv_dx = (float)(1.0 / p->dx);
+  p->inv_2dx = (float)(1.0 / (2.0 * p->dx));
+  p->inv_dx2 = (float)(1.0 / (p->dx * p->dx));
+  const double tw = 2.0 * M_PI / L;
+  p->kscale = (float)(tw * tw);
+  return NSK_OK;
+}
+
+size_t nsk_spectrum_bytes(const nsk_plan* p) {
+  return p ? (size_t)p->batch * p->N * (p->N / 2) * 16 : 0;
+}
+
+size_t nsk_partials_per_slot(const nsk_plan* p, int steps) {
+  return p ? (size_t)(steps > 0 ? steps : 0) * row_bands(p->N) : 0;
+}
+
+int nsk_grad(const nsk_plan* p, const float* f, float* gx, float* gy, void* stream) {
+  CHECK_PLAN(p);
+  const long long NN = (long long)p->N * p->N;
+  return stencil(p, 0, f, nullptr, nullptr, gx, gy, NN, NN, nullptr, 0.f, (hipStream_t)stream);
+}
+
+int nsk_laplacian(const nsk_plan* p, const float* f, float* out, void* stream) {
+  CHECK_PLAN(p);
+  const long long NN = (long long)p->N * p->N;
+  return stencil(p, 1, f, nullptr, nullptr, out, nullptr, NN, NN, nullptr, 0.f, (hipStream_t)stream);
+}
+
+int nsk_divergence(const nsk_plan* p, const float* u, float* out, void* stream) {
+  CHECK_PLAN(p);
+  const long long NN = (long long)p->N * p->N;
+  return stencil(p, 2, u, nullptr, nullptr, out, nullptr, 2 * NN, NN, nullptr, 0.f, (hipStream_t)stream);
+}
+
+int nsk_advect(const nsk_plan* p, const float* field, const float* ux, const float* uy, float* out,
+               int scheme, void* stream) {
+  CHECK_PLAN(p);
+  if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
+    return fail(NSK_E_INVAL, "unknown advection scheme %d", scheme);
+  const long long NN = (long long)p->N * p->N;
+  return stencil(p, scheme == NSK_SCHEME_CENTRAL ? 3 : 4, field, ux, uy, out, nullptr, NN, NN, nullptr,
+                 0.f, (hipStream_t)stream);
+}
+
+int nsk_forcing(const nsk_plan* p, const float* theta2, float t, float* out, void* stream) {
+  CHECK_PLAN(p);
+  if (p->batch != 1) return fail(NSK_E_INVAL, "nsk_forcing takes batch 1");
+  const long long NN = (long long)p->N * p->N;
+  return stencil(p, 5, nullptr, nullptr, nullptr, out, nullptr, 0, 2 * NN, theta2, t, (hipStream_t)stream);
+}
+
+int nsk_poisson(const nsk_plan* p, const float* rhs, float* out, void* spec, void* stream) {
+  CHECK_PLAN(p);
+  if (!rhs || !out || !spec) return fail(NSK_E_INVAL, "nsk_poisson: NULL buffer");
+  hipStream_t st = (hipStream_t)stream;
+  const long long NN = (long long)p->N * p->N;
+  RowArgs r = row_args(p, nullptr);
+  r.src_real = rhs;
+  r.src_bstride = NN;
+  r.src_nc = 1;
+  r.dst_spec = (float4*)spec;
+  HIP_TRY(launch_row(p->N, RK_REAL_TO_SPEC, 0, r, p->batch, st));
+  HIP_TRY(launch_col(p->N, COL_POISSON, col_args(p, spec, nullptr, 0, REQ_ANY), p->batch, st));
+  RowArgs w = row_args(p, nullptr);
+  w.src_spec = (const float4*)spec;
+  w.dst_real = out;
+  w.dst_bstride = NN;
+  w.dst_nc = 1;
+  HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
+  return NSK_OK;
+}
+
+int nsk_project(const nsk_plan* p, const float* v, float* out, void* spec, void* stream) {
+  CHECK_PLAN(p);
+  if (!v || !out || !spec) return fail(NSK_E_INVAL, "nsk_project: NULL buffer");
+  hipStream_t st = (hipStream_t)stream;
+  const long long NN2 = 2LL * p->N * p->N;
+  RowArgs r = row_args(p, nullptr);
+  r.src_real = v;
+  r.src_bstride = NN2;
+  r.dst_spec = (float4*)spec;
+  HIP_TRY(launch_row(p->N, RK_REAL_TO_SPEC, 0, r, p->batch, st));
+  HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, spec, nullptr, 0, REQ_ANY), p->batch, st));
+  RowArgs w = row_args(p, nullptr);
+  w.src_spec = (const float4*)spec;
+  w.dst_real = out;
+  w.dst_bstride = NN2;
+  HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
+  return NSK_OK;
+}
+
+int nsk_step(const nsk_plan* p, const float* u, float* out, const float* force, const nsk_slot* slots,
+             float t, int scheme, void* spec, void* stream) {
+  CHECK_PLAN(p);
+  if (!u || !out || !spec || !slots) return fail(NSK_E_INVAL, "nsk_step: NULL buffer");
+  if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
+    return fail(NSK_E_INVAL, "unknown advection scheme %d", scheme);
+  hipStream_t st = (hipStream_t)stream;
+  const long long NN2 = 2LL * p->N * p->N;
+  RowArgs r = row_args(p, slots);
+  r.src_real = u;
+  r.src_bstride = NN2;
+  r.dst_spec = (float4*)spec;
+  r.force = force;
+  r.force_bstride = NN2;
+  r.t_explicit = t;
+  HIP_TRY(launch_row(p->N, force ? RK_STEP_FORCE : RK_STEP_FROM_REAL, scheme, r, p->batch, st));
+  HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, spec, slots, 0, REQ_ANY), p->batch, st));
+  RowArgs w = row_args(p, nullptr);
+  w.src_spec = (const float4*)spec;
+  w.dst_real = out;
+  w.dst_bstride = NN2;
+  HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
+  return NSK_OK;
+}
+
+static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
+                        int scheme, int objective, double* J_out, void* spec0, void* spec1,
+                        double* jpart, float* ckpt, int ckpt_every, bool use_graph, hipStream_t st) {
+  const long long NN2 = 2LL * p->N * p->N;
+  if (steps == 0) {
+    HIP_TRY(hipMemcpyAsync(u_final, u0, sizeof(float) * NN2 * p->batch, hipMemcpyDeviceToDevice, st));
+    const size_t nb = (size_t)p->batch;
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, st, J_out, nb);
+    HIP_TRY(hipGetLastError());
+    return NSK_OK;
+  }
+  const int ckpt_cap = ckpt ? (steps + ckpt_every - 1) / ckpt_every : 0;
+  // step 0: predictor from the real initial state
+  RowArgs r = row_args(p, slots);
+  r.src_real = u0;
+  r.src_bstride = NN2;
+  r.dst_spec = (float4*)spec0;
+  r.require = REQ_ACTIVE;
+  r.part = jpart;
+  r.pcap = steps;
+  r.objective = objective;
+  if (ckpt) {
+    r.ustore = ckpt;
+    r.st_bstride = (long long)ckpt_cap * NN2;
+    r.st_every = ckpt_every;
+    r.st_cap = ckpt_cap;
+  }
+  HIP_TRY(launch_row(p->N, RK_STEP_FROM_REAL, scheme, r, p->batch, st));
+  void* cur = spec0;
+  void* nxt = spec1;
+  int rc = run_steps(p, slots, steps, scheme, objective, &cur, &nxt, u_final, jpart, steps, ckpt,
+                     ckpt_every, ckpt_cap, use_graph, st);
+  if (rc) return rc;
+  if (J_out) HIP_TRY(finalize(p, slots, jpart, steps, 1, J_out, st));
+  return NSK_OK;
+}
+
+static int check_common(const nsk_plan* p, int steps, int scheme, int objective) {
+  if (steps < 0) return fail(NSK_E_INVAL, "steps must be >= 0, got %d", steps);
+  if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
+    return fail(NSK_E_INVAL, "unknown advection scheme %d", scheme);
+  if (objective != NSK_OBJ_WORK && objective != NSK_OBJ_MEAN_KE)
+    return fail(NSK_E_INVAL, "unknown objective %d", objective);
+  return NSK_OK;
+}
+
+int nsk_rollout(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
+                int scheme, int objective, double* J_out, void* spec0, void* spec1, double* jpart,
+                float* ckpt, int ckpt_every, int use_graph, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, steps, scheme, objective);
+  if (rc) return rc;
+  if (!u0 || !u_final || !slots || !spec0 || !spec1 || (steps > 0 && !jpart))
+    return fail(NSK_E_INVAL, "nsk_rollout: NULL buffer");
+  if (ckpt && ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
+  return rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
+                      ckpt_every, use_graph != 0, (hipStream_t)stream);
+}
+
+int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
+                     int scheme, int objective, double* J_out, double* dtheta_out, float* du0_out,
+                     void* spec0, void* spec1, void* spec2, void* spec3, double* jpart, double* gpart,
+                     float* ckpt, float* seg, int ckpt_every, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, steps, scheme, objective);
+  if (rc) return rc;
+  if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_grad needs steps >= 1");
+  if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
+  if (!u0 || !u_final || !slots || !spec0 || !spec1 || !spec2 || !spec3 || !jpart || !gpart || !ckpt ||
+      !seg || !dtheta_out)
+    return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
+  hipStream_t st = (hipStream_t)stream;
+  const long long NN2 = 2LL * p->N * p->N;
+  const int ncheck = (steps + ckpt_every - 1) / ckpt_every;
+  rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
+                    ckpt_every, false, st);
+  if (rc) return rc;
+  // adjoint: lambda_steps = 0
+  HIP_TRY(zero_spec(p, spec2, st));
+  void* lam = spec2;
+  void* lam_next = spec3;
+  for (int sg = ncheck - 1; sg >= 0; --sg) {
+    const int s0 = sg * ckpt_every;
+    const int s1 = (s0 + ckpt_every < steps) ? s0 + ckpt_every : steps;
+    // recompute u_n, n in [s0, s1), into seg
+    HIP_TRY(set_step(slots, p->batch, s0, st));
+    RowArgs r = row_args(p, slots);
+    r.src_real = ckpt + (long long)sg * NN2;
+    r.src_bstride = (long long)ncheck * NN2;
+    r.dst_spec = (float4*)spec0;
+    r.require = REQ_ACTIVE;
+    r.ustore = seg;
+    r.st_bstride = (long long)ckpt_every * NN2;
+    r.st_base = s0;
+    r.st_every = 1;
+    r.st_cap = ckpt_every;
+    HIP_TRY(launch_row(p->N, RK_STEP_FROM_REAL, scheme, r, p->batch, st));
+    void* cur = spec0;
+    void* nxt = spec1;
+    for (int m = s0; m + 1 < s1; ++m) {
+      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, cur, slots, m - s0, REQ_ACTIVE), p->batch, st));
+      RowArgs q = row_args(p, slots);
+      q.src_spec = (const float4*)cur;
+      q.dst_spec = (float4*)nxt;
+      q.dst_real = u_final;  // never written: n < s1 <= steps
+      q.dst_bstride = NN2;
+      q.step_off = m + 1 - s0;
+      q.require = REQ_ROLL;
+      q.ustore = seg;
+      q.st_bstride = (long long)ckpt_every * NN2;
+      q.st_base = s0;
+      q.st_every = 1;
+      q.st_cap = ckpt_every;
+      HIP_TRY(launch_row(p->N, RK_STEP_ROLL, scheme, q, p->batch, st));
+      std::swap(cur, nxt);
+    }
+    // adjoint sweep over the segment
+    HIP_TRY(set_step(slots, p->batch, 0, st));
+    for (int n = s1 - 1; n >= s0; --n) {
+      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, 0, REQ_ANY), p->batch, st));
+      RowArgs a = row_args(p, slots);
+      a.src_spec = (const float4*)lam;
+      a.dst_spec = (float4*)lam_next;
+      a.step_off = n;
+      a.require = REQ_ANY;
+      a.part = gpart;
+      a.pcap = steps;
+      a.objective = objective;
+      a.ustate = seg;
+      a.us_bstride = (long long)ckpt_every * NN2;
+      a.us_base = s0;
+      HIP_TRY(launch_row(p->N, RK_ADJ, scheme, a, p->batch, st));
+      std::swap(lam, lam_next);
+    }
+  }
+  HIP_TRY(set_step(slots, p->batch, steps, st));
+  if (du0_out) {
+    RowArgs w = row_args(p, nullptr);
+    w.src_spec = (const float4*)lam;
+    w.dst_real = du0_out;
+    w.dst_bstride = NN2;
+    HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
+  }
+  HIP_TRY(finalize(p, slots, gpart, steps, 2, dtheta_out, st));
+  return NSK_OK;
+}
+
+int nsk_admit(const nsk_plan* p, const float* u0, nsk_slot* slots, int scheme, int objective, void* spec,
+              double* jpart, int jcap, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, 0, scheme, objective);
+  if (rc) return rc;
+  if (!u0 || !slots || !spec) return fail(NSK_E_INVAL, "nsk_admit: NULL buffer");
+  RowArgs r = row_args(p, slots);
+  r.src_real = u0;
+  r.src_bstride = 2LL * p->N * p->N;
+  r.dst_spec = (float4*)spec;
+  r.require = REQ_ACTIVE;
+  r.part = jpart;
+  r.pcap = jcap;
+  r.objective = objective;
+  HIP_TRY(launch_row(p->N, RK_STEP_FROM_REAL, scheme, r, p->batch, (hipStream_t)stream));
+  return NSK_OK;
+}
+
+int nsk_advance(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective, void* spec_a,
+                void* spec_b, float* u_final, double* jpart, int jcap, int use_graph, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, chunk, scheme, objective);
+  if (rc) return rc;
+  if (!slots || !spec_a || !spec_b || !u_final) return fail(NSK_E_INVAL, "nsk_advance: NULL buffer");
+  return run_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart, jcap, nullptr, 1, 0,
+                   use_graph != 0 && chunk > 0, (hipStream_t)stream);
+}
+
+int nsk_finalize(const nsk_plan* p, const nsk_slot* slots, const double* part, int pcap, int ncomp,
+                 double* out, void* stream) {
+  CHECK_PLAN(p);
+  if (ncomp != 1 && ncomp != 2) return fail(NSK_E_INVAL, "ncomp must be 1 or 2");
+  if (!slots || !part || !out) return fail(NSK_E_INVAL, "nsk_finalize: NULL buffer");
+  HIP_TRY(finalize(p, slots, part, pcap, ncomp, out, (hipStream_t)stream));
+  return NSK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// nsk_internal.h - argument blocks and per-N launchers shared between the
+// kernel translation units (one per grid size) and the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/nsk.h"
+
+namespace nsk {
+
+enum Src { SRC_REAL = 0, SRC_SPEC = 1 };
+enum Op { OP_COPY = 0, OP_STEP = 1, OP_ADJ = 2 };
+enum Dst { DST_REAL = 0, DST_SPEC = 1 };
+enum ColMode { COL_PROJECT = 0, COL_POISSON = 1, COL_IDENT = 2 };
+// Which slots a launch touches, given n = slots[b].step + step_off.
+enum Require {
+  REQ_ANY = 0,     // every slot (operator-level calls)
+  REQ_ROLL = 1,    // 0 <= n < nsteps: fused step; n == nsteps: write real state
+  REQ_ACTIVE = 2,  // 0 <= n < nsteps only
+};
+
+struct RowArgs {
+  nsk_plan plan;
+  const float* src_real;  // [B][src_nc][N][N]
+  long long src_bstride;  // floats between slots
+  int src_nc;             // 1: second component is zero
+  const float4* src_spec; // [B][N][N/2]
+  float* dst_real;        // [B][dst_nc][N][N]
+  long long dst_bstride;
+  int dst_nc;
+  float4* dst_spec;       // [B][N][N/2]
+  const float* force;     // nullable explicit force [B][2][N][N]
+  long long force_bstride;
+  const nsk_slot* slots;
+  int step_off;
+  int require;
+  float t_explicit;       // >= 0: time used instead of float(n)*dt
+  // objective partials (forward) / theta-gradient partials (adjoint)
+  double* part;           // [B][pcap][nbands] (x2 for the adjoint)
+  int pcap;
+  int objective;
+  // forward: store u_n into ustore[(n - st_base)/st_every] when it divides
+  float* ustore;
+  long long st_bstride;
+  int st_base, st_every, st_cap;
+  // adjoint: forward state u_n at ustate[n - us_base]
+  const float* ustate;
+  long long us_bstride;
+  int us_base;
+};
+
+struct ColArgs {
+  nsk_plan plan;
+  float4* spec;  // [B][N][N/2], transformed in place
+  const nsk_slot* slots;
+  int step_off;
+  int require;   // REQ_ANY or REQ_ACTIVE
+};
+
+struct StencilArgs {
+  nsk_plan plan;
+  const float* a;  // primary input
+  const float* b;  // secondary inputs (advect: ux, uy)
+  const float* c;
+  float* out0;
+  float* out1;
+  long long in_bstride, out_bstride;
+  int op;      // 0 grad, 1 laplacian, 2 divergence, 3 advect central, 4 advect upwind, 5 forcing
+  const float* theta;
+  float t;
+};
+
+// Row-pass geometry for grid size N: G rows per iteration, ITER iterations
+// per workgroup, 256 threads.
+template <int N>
+struct RowCfg;
+template <> struct RowCfg<4096> { static constexpr int G = 1, ITER = 16; };
+template <> struct RowCfg<2048> { static constexpr int G = 2, ITER = 4; };
+template <> struct RowCfg<1024> { static constexpr int G = 4, ITER = 2; };
+template <> struct RowCfg<512> { static constexpr int G = 8, ITER = 1; };
+template <> struct RowCfg<256> { static constexpr int G = 16, ITER = 1; };
+template <> struct RowCfg<128> { static constexpr int G = 32, ITER = 1; };
+template <> struct RowCfg<64> { static constexpr int G = 64, ITER = 1; };
+template <> struct RowCfg<32> { static constexpr int G = 32, ITER = 1; };
+template <> struct RowCfg<16> { static constexpr int G = 16, ITER = 1; };
+template <> struct RowCfg<8> { static constexpr int G = 8, ITER = 1; };
+
+// Column-pass geometry: W spectral columns (two fields each) per workgroup.
+template <int N>
+struct ColCfg;
+template <> struct ColCfg<4096> { static constexpr int W = 2, T = 512; };
+template <> struct ColCfg<2048> { static constexpr int W = 2, T = 512; };
+template <> struct ColCfg<1024> { static constexpr int W = 2, T = 256; };
+template <> struct ColCfg<512> { static constexpr int W = 2, T = 256; };
+template <> struct ColCfg<256> { static constexpr int W = 16, T = 512; };
+template <> struct ColCfg<128> { static constexpr int W = 32, T = 512; };
+template <> struct ColCfg<64> { static constexpr int W = 32, T = 512; };
+template <> struct ColCfg<32> { static constexpr int W = 16, T = 256; };
+template <> struct ColCfg<16> { static constexpr int W = 8, T = 256; };
+template <> struct ColCfg<8> { static constexpr int W = 4, T = 256; };
+
+constexpr int ROW_T = 256;
+
+inline int row_bands(int N) {
+  switch (N) {
+    case 4096: return 4096 / (RowCfg<4096>::G * RowCfg<4096>::ITER);
+    case 2048: return 2048 / (RowCfg<2048>::G * RowCfg<2048>::ITER);
+    case 1024: return 1024 / (RowCfg<1024>::G * RowCfg<1024>::ITER);
+    case 512: return 512 / RowCfg<512>::G;
+    case 256: return 256 / RowCfg<256>::G;
+    case 128: return 128 / RowCfg<128>::G;
+    default: return 1;
+  }
+}
+
+// Variant selector for the row kernel.
+enum RowKind {
+  RK_REAL_TO_SPEC = 0,    // copy: real pair -> spectrum
+  RK_SPEC_TO_REAL = 1,    // copy: spectrum -> real pair
+  RK_STEP_FROM_REAL = 2,  // predictor from a real state, theta forcing
+  RK_STEP_FORCE = 3,      // predictor from a real state, explicit force
+  RK_STEP_ROLL = 4,       // inverse row FFT + predictor + forward row FFT
+  RK_ADJ = 5,             // adjoint of the predictor on the row-spectral form
+};
+
+// Per-N launchers (defined in the per-N kernel objects).
+#define NSK_DECLARE_LAUNCHERS(N)                                                   \
+  hipError_t launch_row_##N(int kind, int scheme, const RowArgs& a, int batch,     \
+                            hipStream_t s);                                        \
+  hipError_t launch_col_##N(int mode, const ColArgs& a, int batch, hipStream_t s); \
+  hipError_t launch_init_tw_##N(hipStream_t s);
+NSK_DECLARE_LAUNCHERS(8)
+NSK_DECLARE_LAUNCHERS(16)
+NSK_DECLARE_LAUNCHERS(32)
+NSK_DECLARE_LAUNCHERS(64)
+NSK_DECLARE_LAUNCHERS(128)
+NSK_DECLARE_LAUNCHERS(256)
+NSK_DECLARE_LAUNCHERS(512)
+NSK_DECLARE_LAUNCHERS(1024)
+NSK_DECLARE_LAUNCHERS(2048)
+NSK_DECLARE_LAUNCHERS(4096)
+
+}  // namespace nsk

@@ -1,0 +1,168 @@
+"""Operator-level parity of the HIP kernels against the CPU oracle
+(reference NS_solver.py:206-265, 474-477, 517-527, 621-640, 671-697).
+
+Tolerances: stencils and forcing are elementwise float32 arithmetic, so they
+are compared with rtol=1e-5 / atol scaled to the field; FFT-based results
+(Poisson, projection, step) to a relative L2 error of 1e-5 against FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_oracle as AO
+from oracle import ns_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+def S():
+    from ns_solver_jax_differential_distributed_amd import NS_solver
+    return NS_solver
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def field(N, seed, scale=1.0, comps=None):
+    rng = np.random.default_rng(seed)
+    shape = (N, N) if comps is None else (comps, N, N)
+    return (scale * rng.standard_normal(shape)).astype(np.float32)
+
+
+def smooth_field(N, seed, kmax=6, L=1.0, comps=2):
+    """Band-limited random field, |k| <= kmax (keeps FD/spectral results O(1))."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((comps, N, N))
+    kmax = min(kmax, N // 2 - 1)
+    for c in range(comps):
+        C = np.zeros((N, N), np.complex128)
+        for i in range(-kmax, kmax + 1):
+            for j in range(-kmax, kmax + 1):
+                a, b = rng.standard_normal(2) / (1 + i * i + j * j)
+                C[i % N, j % N] += a + 1j * b
+        out[c] = (np.fft.ifft2(C) * N * N).real
+    return out.astype(np.float32)
+
+
+@pytest.mark.parametrize("N", [8, 16, 64, 256])
+@pytest.mark.parametrize("L", [1.0, 2 * math.pi])
+def test_stencils_match_oracle(N, L):
+    dx = L / N
+    ops = S().PeriodicDifferentialOps(dx)
+    f = field(N, 1)
+    u = field(N, 2, comps=2)
+    gx, gy = ops.grad(torch.from_numpy(f))
+    ogx, ogy = O.grad(f.astype(np.float64), dx)
+    sc = 1.0 / dx
+    np.testing.assert_allclose(gx.cpu().numpy(), ogx, rtol=1e-5, atol=1e-6 * sc)
+    np.testing.assert_allclose(gy.cpu().numpy(), ogy, rtol=1e-5, atol=1e-6 * sc)
+    lap = ops.laplacian(torch.from_numpy(f)).cpu().numpy()
+    np.testing.assert_allclose(lap, O.laplacian(f.astype(np.float64), dx), rtol=1e-5, atol=1e-5 * sc * sc)
+    div = ops.divergence(torch.from_numpy(u)).cpu().numpy()
+    np.testing.assert_allclose(div, O.divergence(u.astype(np.float64), dx), rtol=1e-5, atol=1e-6 * sc)
+    for scheme in ("central", "upwind"):
+        adv = S().make_advection_scheme(scheme).advect(
+            torch.from_numpy(f), torch.from_numpy(u[0]), torch.from_numpy(u[1]), ops).cpu().numpy()
+        ref = O.advect(f.astype(np.float64), u[0].astype(np.float64), u[1].astype(np.float64), dx, scheme)
+        np.testing.assert_allclose(adv, ref, rtol=1e-5, atol=1e-5 * sc)
+
+
+def test_upwind_tie_uses_backward_difference():
+    """u == 0 takes the backward branch (jnp.where(u >= 0, ...), NS_solver.py:261,264)."""
+    N, dx = 16, 1.0 / 16
+    ops = S().PeriodicDifferentialOps(dx)
+    f = field(N, 3)
+    ux = np.zeros((N, N), np.float32)
+    uy = np.zeros((N, N), np.float32)
+    ux[3, 4] = 0.0
+    ux[5, 5] = -0.0
+    uy[7, 1] = 1.0
+    got = ops.upwind_advect(torch.from_numpy(f), torch.from_numpy(ux), torch.from_numpy(uy)).cpu().numpy()
+    np.testing.assert_allclose(got, O.upwind_advect(f, ux, uy, dx), rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("L", [1.0, 2 * math.pi])
+@pytest.mark.parametrize("t", [0.0, 0.137, 0.5, 3.3])
+def test_forcing_matches_oracle(L, t):
+    N = 128
+    grid = S().make_grid(N, L)
+    phys = S().Phys(nu=0.01, mu0=0.5, sigma0=0.1, C=1.0)
+    f = S().GaussianForcingField(grid, phys)([0.9, 0.4, 1.3], t).cpu().numpy()
+    ref = O.forcing([0.9, 0.4], t, N, L, O.PhysParams(0.01), np.float32)
+    np.testing.assert_allclose(f, ref, rtol=2e-6, atol=2e-7)
+    assert np.all(f[1] == 0)
+
+
+@pytest.mark.parametrize("N", SIZES)
+def test_poisson_matches_oracle(N):
+    L = 1.0
+    dx = L / N
+    rhs = smooth_field(N, 5, kmax=min(6, N // 2 - 1), comps=1)[0]
+    rhs -= rhs.mean()
+    p = S().SpectralPoissonSolver(N, N, dx)(torch.from_numpy(rhs)).cpu().numpy()
+    ref = O.poisson(rhs.astype(np.float64), dx)
+    assert rel_l2(p, ref) < 1e-5
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_poisson_manufactured_solution(N):
+    """Reference unit_tests/test_distributed_solver.py:372-396."""
+    L = 1.0
+    dx = L / N
+    x = np.linspace(0.0, L, N, endpoint=False)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    p_exact = np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    rhs = -2.0 * (2.0 * np.pi) ** 2 * p_exact
+    p = S().SpectralPoissonSolver(N, N, dx)(torch.tensor(rhs, dtype=torch.float32)).cpu().numpy()
+    assert np.max(np.abs(p - p_exact)) < 2e-4
+
+
+@pytest.mark.parametrize("N", SIZES)
+def test_projection_matches_oracle(N):
+    L = 2 * math.pi
+    dx = L / N
+    v = field(N, 11, comps=2) if N <= 64 else smooth_field(N, 11, kmax=8, L=L)
+    got = S().SpectralPoissonSolver(N, N, dx).project(torch.from_numpy(v)).cpu().numpy()
+    ref = AO.projection(v.astype(np.float64), dx)
+    assert rel_l2(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("N", [16, 64, 256, 1024, 4096])
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+def test_step_matches_oracle(N, scheme):
+    L = 2 * math.pi
+    dx = L / N
+    u = smooth_field(N, 21, kmax=8, L=L)
+    nu, dt, t, theta = 1e-3, 0.2 * dx, 0.31, [1.1, 0.7, 0.9]
+    grid = S().make_grid(N, L)
+    phys = S().Phys(nu=nu, mu0=0.5, sigma0=0.1, C=1.0)
+    solver = S().SingleCoreNSSolver(grid, phys, S().Time(dt=dt, steps=1), advection_scheme=scheme)
+    got = solver.step(torch.from_numpy(u), theta, t).cpu().numpy()
+    f = O.forcing(theta, np.float32(t), N, L, O.PhysParams(nu), np.float64)
+    ref = O.step(u.astype(np.float64), f, dx, nu, dt, scheme)
+    assert rel_l2(got, ref) < 1e-5
+    # explicit-force path of ChorinProjector.step
+    ops = S().PeriodicDifferentialOps(dx)
+    proj = S().ChorinProjector(ops, S().SpectralPoissonSolver(N, N, dx),
+                               S().make_advection_scheme(scheme), nu, dt)
+    got2 = proj.step(torch.from_numpy(u), torch.tensor(f, dtype=torch.float32)).cpu().numpy()
+    assert rel_l2(got2, ref) < 1e-5
+
+
+def test_step_divergence_free():
+    """Reference unit_tests/test_distributed_solver.py:267-285 (N=32, u0=0)."""
+    N = 32
+    grid = S().make_grid(N, 1.0)
+    phys = S().Phys(nu=0.01, mu0=0.5, sigma0=0.1, C=1.0)
+    solver = S().SingleCoreNSSolver(grid, phys, S().Time(dt=0.001, steps=1))
+    u1 = solver.step(torch.zeros(2, N, N), [0.9, 0.4], 0.0).cpu().numpy()
+    gx = (np.roll(u1[0], -1, 0) - np.roll(u1[0], 1, 0)) / (2 * grid.dx)
+    gy = (np.roll(u1[1], -1, 1) - np.roll(u1[1], 1, 1)) / (2 * grid.dx)
+    assert np.max(np.abs(gx + gy)) < 5e-4

@@ -1,0 +1,151 @@
+"""Rollout, objective and adjoint gradient of the HIP path against the
+oracle and the reference's committed results (NS_solver.py:732-756)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_oracle as AO
+from oracle import ns_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def S():
+    from ns_solver_jax_differential_distributed_amd import NS_solver
+    return NS_solver
+
+
+def gold():
+    with open(os.path.join(GOLD, "reference_results.json")) as f:
+        return json.load(f)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def solver(N, L, nu, dt, steps, scheme="central", **kw):
+    grid = S().make_grid(N, L)
+    phys = S().Phys(nu=nu, mu0=0.5, sigma0=0.1, C=1.0)
+    return S().SingleCoreNSSolver(grid, phys, S().Time(dt=dt, steps=steps), advection_scheme=scheme, **kw)
+
+
+def test_golden_speed_field_N128():
+    """Runner output for sample_config/config_N_128.json: N=128, nu=1e-4,
+    t=0.5 (160 steps), central, theta=[1,0.5], u0=0 (runner.py:115-125)."""
+    N, nu = 128, 1e-4
+    dt = O.config_dt(N, 1.0, nu)
+    steps = O.config_steps(0.5, dt)
+    s = solver(N, 1.0, nu, dt, steps)
+    J, u = s.rollout(torch.zeros(2, N, N), [1.0, 0.5])
+    speed = torch.sqrt(u[0] ** 2 + u[1] ** 2).cpu().numpy()
+    ref = np.load(os.path.join(GOLD, "speed_single_N128.npy"))
+    assert np.max(np.abs(speed - ref)) < 2e-6
+
+
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+@pytest.mark.parametrize("nu", [0.005, 0.01, 0.05])
+def test_golden_scheme_comparison(scheme, nu):
+    """J, final kinetic energy and max |u| of the reference's scheme study
+    (studies.py:545-606): N=128, L=1, t=0.5, theta=[1,0.5], u0=0."""
+    g = gold()["scheme_comparison"]
+    key = f"{scheme}_nu_{nu:.4f}"
+    N = 128
+    dt = g[key + "_dt"]
+    steps = O.config_steps(0.5, dt)
+    s = solver(N, 1.0, nu, dt, steps, scheme)
+    J, u = s.rollout(torch.zeros(2, N, N), [1.0, 0.5])
+    un = u.cpu().numpy().astype(np.float64)
+    assert abs(float(J) - g[key + "_objective"]) / g[key + "_objective"] < 2e-6
+    assert abs(O.kinetic_energy(un, 1.0 / N) - g[key + "_kinetic_energy"]) / g[key + "_kinetic_energy"] < 1e-5
+    assert abs(O.speed(un).max() - g[key + "_max_velocity"]) / g[key + "_max_velocity"] < 1e-5
+
+
+def test_golden_speed_field_N512():
+    """config_N_512.json runner output (640 steps), per-row mean of |u|."""
+    N, nu = 512, 1e-4
+    dt = O.config_dt(N, 1.0, nu)
+    steps = O.config_steps(0.5, dt)
+    s = solver(N, 1.0, nu, dt, steps)
+    _, u = s.rollout(torch.zeros(2, N, N), [1.0, 0.5])
+    speed = torch.sqrt(u[0] ** 2 + u[1] ** 2).double().mean(dim=1).cpu().numpy()
+    ref = np.array(gold()["speed_N512"]["row_mean"])
+    assert np.max(np.abs(speed - ref)) < 2e-6
+
+
+@pytest.mark.parametrize("N,steps", [(16, 30), (64, 25), (256, 12)])
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+@pytest.mark.parametrize("objective", ["work", "mean_ke"])
+def test_rollout_matches_fp64_oracle(N, steps, scheme, objective):
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    u0 = W.initial_velocity(N, seed=3, kmax=4)
+    nu = 1e-3
+    dt = W.cfl_dt(u0, L, 0.3, nu)
+    theta = [1.2, 0.8, 0.6]
+    s = solver(N, L, nu, dt, steps, scheme, objective=objective)
+    J, u = s.rollout(torch.from_numpy(u0), theta)
+    Jo, uo = O.rollout(u0.astype(np.float64), theta, N, L, O.PhysParams(nu), dt, steps, scheme, objective)
+    assert rel_l2(u.cpu().numpy(), uo) < 1e-5
+    assert abs(float(J) - Jo) / abs(Jo) < 1e-5
+
+
+def test_rollout_graph_and_eager_agree_bitwise():
+    N, steps = 64, 70
+    rng = np.random.default_rng(0)
+    u0 = (0.2 * rng.standard_normal((2, N, N))).astype(np.float32)
+    a = solver(N, 1.0, 0.01, 1e-3, steps, use_graph=True)
+    b = solver(N, 1.0, 0.01, 1e-3, steps, use_graph=False)
+    Ja, ua = a.rollout(torch.from_numpy(u0), [0.9, 0.4])
+    Jb, ub = b.rollout(torch.from_numpy(u0), [0.9, 0.4])
+    Ja2, ua2 = a.rollout(torch.from_numpy(u0), [0.9, 0.4])
+    assert torch.equal(ua, ub) and torch.equal(ua, ua2)
+    assert float(Ja) == float(Jb) == float(Ja2)
+
+
+def test_zero_steps_returns_input():
+    s = solver(16, 1.0, 0.01, 1e-3, 0)
+    u0 = torch.randn(2, 16, 16)
+    J, u = s.rollout(u0, [1.0, 0.5])
+    assert float(J) == 0.0 and torch.equal(u.cpu(), u0)
+
+
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+@pytest.mark.parametrize("objective", ["work", "mean_ke"])
+def test_gradient_matches_adjoint_oracle(scheme, objective):
+    N, L, nu, dt, steps = 32, 1.0, 0.01, 1e-3, 23
+    rng = np.random.default_rng(4)
+    u0 = (0.4 * rng.standard_normal((2, N, N))).astype(np.float32)
+    theta = [0.9, 0.4, 1.7]
+    s = solver(N, L, nu, dt, steps, scheme, objective=objective, ckpt_every=5)
+    J, u, g, du0 = s.rollout_and_grad(torch.from_numpy(u0), theta)
+    Jo, uo, go, lo = AO.rollout_gradient(u0.astype(np.float64), theta, N, L, O.PhysParams(nu), dt, steps,
+                                         scheme, objective)
+    gg = g.cpu().numpy()
+    assert gg[2] == 0.0
+    assert np.max(np.abs(gg[:2] - go[:2])) / np.max(np.abs(go[:2])) < 1e-4
+    assert rel_l2(du0.cpu().numpy(), lo) < 1e-4
+    assert abs(float(J) - Jo) / abs(Jo) < 1e-5
+
+
+def test_jit_gradient_matches_jax_reverse_mode():
+    """JAX's gradient at theta=[1,0.5] from the reference's SGD(lr=0.1)
+    trajectory (studies.py:890-918): g = (theta_0 - theta_1)/0.1;
+    default config N=128, nu=0.01, 328 steps, u0=0."""
+    p = np.array(gold()["sgd_lr0.1"]["parameters"])
+    g_jax = (p[0] - p[1]) / 0.1
+    N, nu = 128, 0.01
+    dt = O.config_dt(N, 1.0, nu)
+    steps = O.config_steps(0.5, dt)
+    s = solver(N, 1.0, nu, dt, steps)
+    g = s.jit_gradient(np.array([1.0, 0.5], np.float32)).cpu().numpy()
+    # theta_1 is stored in float32: |g_jax error| <= ulp(1)/2/0.1 = 6e-7
+    assert abs(g[1] - g_jax[1]) < 2e-6
+    assert abs(g[0] - g_jax[0]) < 2e-6

@@ -1,0 +1,240 @@
+"""Host-side logic that runs without a GPU: the reference's config and error
+contracts (unit_tests/test_distributed_solver.py:462-562), the C ABI library
+(loads, exports every symbol of include/nsk.h, host-only plan setup), and a
+NumPy model of the row-spectral layout the kernels use."""
+import ctypes
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+from ns_solver_jax_differential_distributed_amd import NS_solver as S  # noqa: E402
+from ns_solver_jax_differential_distributed_amd import _lib  # noqa: E402
+from oracle import adjoint_oracle as AO  # noqa: E402
+from oracle import ns_oracle as O  # noqa: E402
+
+
+# ---------------------------------------------------------------- SimConfig / load_config
+@pytest.mark.parametrize("kw", [dict(N=-1), dict(N=0), dict(nu=0.0), dict(nu=-0.001),
+                                dict(advection_scheme="quickscheme"), dict(advection_scheme="")])
+def test_simconfig_rejects_bad_values(kw):
+    with pytest.raises(ValueError):
+        S.SimConfig(**kw)
+
+
+def _write(tmp_path, payload):
+    p = tmp_path / "cfg.json"
+    p.write_text(json.dumps(payload))
+    return str(p)
+
+
+def test_load_config_defaults_and_overrides(tmp_path):
+    assert S.load_config(None).N == 128
+    assert S.load_config("/nonexistent/path/does_not_exist.json").N == 128
+    cfg = S.load_config(_write(tmp_path, {"N": 64, "nu": 0.005, "t": 1.0, "advection_scheme": "upwind"}))
+    assert (cfg.N, cfg.advection_scheme) == (64, "upwind")
+    assert cfg.nu == pytest.approx(0.005) and cfg.t == pytest.approx(1.0)
+    cfg = S.load_config(_write(tmp_path, {"N": 32}))
+    assert cfg.N == 32 and cfg.nu == pytest.approx(S.SimConfig().nu)
+    assert S.load_config(_write(tmp_path, {"N": 48, "unknown_parameter": 999})).N == 48
+    with pytest.raises(ValueError):
+        S.load_config(_write(tmp_path, {"advection_scheme": "invalid"}))
+    np.testing.assert_allclose(S.load_config(_write(tmp_path, {"theta0": [0.3, 0.7]})).theta0,
+                               [0.3, 0.7], atol=1e-6)
+
+
+def test_simconfig_derived_quantities():
+    cfg = S.SimConfig(N=32, t=0.5)
+    assert cfg.steps == int(np.ceil(cfg.t / cfg.dt)) > 0
+    assert S.SimConfig(N=64, L=2.0).dx == pytest.approx(2.0 / 64)
+    cfg = S.SimConfig(N=64, nu=0.01, cfl=0.4, v_max=1.0)
+    assert cfg.dt <= cfg.cfl * cfg.dx / cfg.v_max + 1e-12
+    assert cfg.dt <= cfg.dx ** 2 / (4.0 * cfg.nu) + 1e-12
+    # identical to the oracle's restatement of NS_solver.py:74-79 (and to the
+    # dt the reference studies stored)
+    for nu in (1e-4, 0.005, 0.01, 0.05):
+        assert S.SimConfig(N=128, nu=nu).dt == O.config_dt(128, 1.0, nu)
+    assert S.SimConfig().steps == 328
+
+
+def test_sample_configs_shape_set():
+    """Steps of the sample configs (N in 128..4096, nu=1e-4, t=0.5)."""
+    for N in (128, 256, 512, 1024, 2048, 4096):
+        cfg = S.SimConfig(N=N, nu=1e-4, t=0.5)
+        assert cfg.steps == math.ceil(0.5 / np.float32(0.4 / N)) or cfg.steps == math.ceil(1.25 * N)
+
+
+def test_make_grid_coordinates():
+    g = S.make_grid(16, 2 * math.pi)
+    assert g.X.shape == (16, 16) and g.dx == pytest.approx(2 * math.pi / 16)
+    np.testing.assert_array_equal(g.X[:, 0].numpy(), O.grid_x(16, 2 * math.pi, np.float32))
+    np.testing.assert_array_equal(g.Y[0, :].numpy(), O.grid_x(16, 2 * math.pi, np.float32))
+
+
+def test_scheme_factory_and_abstract_base():
+    assert isinstance(S.make_advection_scheme("central"), S.CentralAdvection)
+    assert isinstance(S.make_advection_scheme("upwind"), S.UpwindAdvection)
+    with pytest.raises(ValueError):
+        S.make_advection_scheme("quick")
+    ops = S.DifferentialOps(0.1)
+    for call in (lambda: ops.grad(None), lambda: ops.laplacian(None),
+                 lambda: ops.upwind_advect(None, None, None),
+                 lambda: S.AdvectionScheme().advect(None, None, None, ops)):
+        with pytest.raises(NotImplementedError):
+            call()
+
+
+@pytest.mark.parametrize("N", [0, 4, 12, 100, 8192])
+def test_unsupported_grid_sizes_raise(N):
+    with pytest.raises(ValueError):
+        S.SpectralPoissonSolver(N, N, 1.0 / max(N, 1))
+
+
+def test_solver_rejects_bad_options():
+    grid = S.make_grid(16, 1.0)
+    phys = S.Phys(0.01, 0.5, 0.1, 1.0)
+    with pytest.raises(ValueError):
+        S.SingleCoreNSSolver(grid, phys, S.Time(1e-3, 1), advection_scheme="bad")
+    with pytest.raises(ValueError):
+        S.SingleCoreNSSolver(grid, phys, S.Time(1e-3, 1), objective="energy")
+
+
+def test_gpu_calls_fail_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError):
+        S.PeriodicDifferentialOps(0.1).laplacian(np.zeros((16, 16), np.float32))
+
+
+# ---------------------------------------------------------------- the C ABI library
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "nsk.h")).read()
+    return sorted(set(re.findall(r"\b(nsk_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load_library()
+    syms = _header_symbols()
+    assert set(syms) == set(_lib.EXPORTED)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_plan_init_host_only():
+    p = _lib.make_plan(256, 2 * math.pi, 0.5, 0.1, 1.0, batch=3)
+    dx = 2 * math.pi / 256
+    assert (p.N, p.batch) == (256, 3)
+    assert p.dx == pytest.approx(dx, rel=1e-15)
+    assert p.inv_2dx == np.float32(1 / (2 * dx)) and p.inv_dx2 == np.float32(1 / dx ** 2)
+    assert p.kscale == np.float32(1.0)
+    lib = _lib.load_library()
+    assert lib.nsk_spectrum_bytes(ctypes.byref(p)) == 3 * 256 * 128 * 16
+    for bad in (12, 4, 8192, 0):
+        with pytest.raises(ValueError):
+            _lib.make_plan(bad, 1.0)
+    with pytest.raises(ValueError):
+        _lib.make_plan(64, -1.0)
+
+
+def test_slot_record_layout():
+    import torch
+    rec = _lib.make_slots(0.01, 1e-3, 0.9, 0.4, 5, 100, batch=2, device=torch.device("cpu"))
+    assert rec.shape == (2, _lib.SLOT_WORDS) and rec.dtype == torch.int32
+    f = rec.view(torch.float32)
+    assert f[1, 0].item() == np.float32(0.01) and f[1, 3].item() == np.float32(0.4)
+    assert rec[0, 4].item() == 5 and rec[0, 5].item() == 100
+
+
+# ---------------------------------------------------------------- row-spectral layout model
+def _row_fwd(u):
+    """Host model of the row pass output: per row, the half spectra of u_x and
+    u_y packed as float4, with ky=0 holding the (real) DC and Nyquist terms."""
+    N = u.shape[-1]
+    HN = N // 2
+    Z = np.fft.fft(u[0] + 1j * u[1], axis=1)
+    H = np.zeros((N, HN, 4))
+    H[:, 0] = np.stack([Z[:, 0].real, Z[:, HN].real, Z[:, 0].imag, Z[:, HN].imag], -1)
+    for c in range(1, HN):
+        zk, zm = Z[:, c], Z[:, N - c]
+        X = (zk + np.conj(zm)) / 2
+        Y = (zk - np.conj(zm)) / 2j
+        H[:, c] = np.stack([X.real, X.imag, Y.real, Y.imag], -1)
+    return H
+
+
+def _row_inv(H):
+    N = H.shape[0]
+    HN = N // 2
+    Z = np.zeros((N, N), complex)
+    Z[:, 0] = H[:, 0, 0] + 1j * H[:, 0, 2]
+    Z[:, HN] = H[:, 0, 1] + 1j * H[:, 0, 3]
+    for c in range(1, HN):
+        h = H[:, c]
+        Z[:, c] = (h[:, 0] - h[:, 3]) + 1j * (h[:, 1] + h[:, 2])
+        Z[:, N - c] = (h[:, 0] + h[:, 3]) + 1j * (h[:, 2] - h[:, 1])
+    z = np.fft.ifft(Z, axis=1)
+    return np.stack([z.real, z.imag])
+
+
+def _col(H, L, mode):
+    N = H.shape[0]
+    HN = N // 2
+    dx = L / N
+
+    def sym(kx, ky):
+        kxs = kx if kx < HN else kx - N
+        kys = ky if ky < HN else ky - N
+        return (np.sin(2 * np.pi * kx / N) / dx, np.sin(2 * np.pi * ky / N) / dx,
+                (2 * np.pi / L) ** 2 * (kxs * kxs + kys * kys))
+
+    def app(X, Y, kx, ky):
+        sx, sy, k2 = sym(kx, ky)
+        if k2 == 0:
+            return (X, Y) if mode == "project" else (0 * X, 0 * Y)
+        if mode == "project":
+            d = sx * X + sy * Y
+            return X - sx * d / k2, Y - sy * d / k2
+        return -X / k2, -Y / k2
+
+    out = np.zeros_like(H)
+    for c in range(HN):
+        CX = np.fft.fft(H[:, c, 0] + 1j * H[:, c, 1])
+        CY = np.fft.fft(H[:, c, 2] + 1j * H[:, c, 3])
+        NX = np.zeros(N, complex)
+        NY = np.zeros(N, complex)
+        if c == 0:
+            for kx in range(HN + 1):
+                k2 = (N - kx) % N
+                AX, BX = (CX[kx] + np.conj(CX[k2])) / 2, (CX[kx] - np.conj(CX[k2])) / 2j
+                AY, BY = (CY[kx] + np.conj(CY[k2])) / 2, (CY[kx] - np.conj(CY[k2])) / 2j
+                AX, AY = app(AX, AY, kx, 0)
+                BX, BY = app(BX, BY, kx, HN)
+                NX[kx], NX[k2] = AX + 1j * BX, np.conj(AX) + 1j * np.conj(BX)
+                NY[kx], NY[k2] = AY + 1j * BY, np.conj(AY) + 1j * np.conj(BY)
+        else:
+            for kx in range(N):
+                NX[kx], NY[kx] = app(CX[kx], CY[kx], kx, c)
+        NX, NY = np.fft.ifft(NX), np.fft.ifft(NY)
+        out[:, c] = np.stack([NX.real, NX.imag, NY.real, NY.imag], -1)
+    return out
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_layout_model_round_trip_projection_poisson(N):
+    """The packed row-spectral form (include/nsk.h) with the column pass's
+    DC/Nyquist separation reproduces the oracle's projection and Poisson
+    solve exactly (FP64)."""
+    L = 2 * math.pi
+    rng = np.random.default_rng(N)
+    u = rng.standard_normal((2, N, N))
+    np.testing.assert_allclose(_row_inv(_row_fwd(u)), u, atol=1e-13)
+    np.testing.assert_allclose(_row_inv(_col(_row_fwd(u), L, "project")), AO.projection(u, L / N), atol=1e-12)
+    r = rng.standard_normal((N, N))
+    p = _row_inv(_col(_row_fwd(np.stack([r, 0 * r])), L, "poisson"))[0]
+    np.testing.assert_allclose(p, O.poisson(r, L / N), atol=1e-12)
