@@ -123,6 +123,20 @@ int nsk_step(const nsk_plan *p, const float *u, float *out, const float *force,
              const nsk_slot *slots, float t, int scheme, void *spec,
              void *stream);
 
+/* ---- single passes (the pressure_solve / advection_diffusion stages that
+ * solver_profiler.py:97-142 times separately) ------------------------------ */
+/* Column pass on a spectrum buffer, in place: column FFT, multiplier, inverse
+ * column FFT.  mode 0: pressure projection I - s s^T/|k|^2 (NS_solver.py:
+ * 691-697); mode 1: Poisson multiplier -1/|k|^2 with the k=0 mode zeroed
+ * (NS_solver.py:523-527).  Both include the 1/N^2 of the inverse FFT. */
+int nsk_column_pass(const nsk_plan *p, void *spec, int mode, void *stream);
+/* Fused row pass of one rollout step for every slot at step n =
+ * slots[b].step + step_off: spectrum of u_n (spec_in) -> objective partial
+ * -> predictor -> spectrum of u*_n (spec_out).  jpart may be NULL. */
+int nsk_row_pass(const nsk_plan *p, const void *spec_in, void *spec_out,
+                 const nsk_slot *slots, int step_off, int scheme, int objective,
+                 double *jpart, int jcap, void *stream);
+
 /* ---- rollout (SingleCoreNSSolver.rollout, NS_solver.py:732-750) ------- */
 /* Runs `steps` Chorin steps for every slot from u0 (slots[b].step must be
  * 0 on entry and is advanced by `steps`).  J_out[b] (double, device)

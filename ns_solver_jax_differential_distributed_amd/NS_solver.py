@@ -539,11 +539,13 @@ class SingleCoreNSSolver:
         """Single time step with the Gaussian forcing at time t (NS_solver.py:727-730)."""
         return self.projector._run(u, None, theta, t)
 
-    def rollout(self, u0, theta):
+    def rollout(self, u0, theta, steps: Optional[int] = None):
         """Integrate self.time.steps steps from u0 accumulating
         J = sum_i dt * sum(u_i . f_i) * dx^2 before each step
-        (NS_solver.py:732-750).  Returns (J, u_final)."""
-        N, steps = self.grid.N, int(self.time.steps)
+        (NS_solver.py:732-750).  Returns (J, u_final).  ``steps`` overrides
+        the step count for this call (extension)."""
+        N = self.grid.N
+        steps = int(self.time.steps if steps is None else steps)
         u0 = _field(u0, (2, N, N))
         A, B = _theta_ab(theta)
         p = self._plan()

@@ -31,7 +31,7 @@ EXPORTED = (
     "nsk_partials_per_slot", "nsk_grad", "nsk_laplacian", "nsk_divergence",
     "nsk_advect", "nsk_forcing", "nsk_poisson", "nsk_project", "nsk_step",
     "nsk_rollout", "nsk_rollout_grad", "nsk_admit", "nsk_advance",
-    "nsk_finalize",
+    "nsk_finalize", "nsk_column_pass", "nsk_row_pass",
 )
 
 
@@ -75,6 +75,8 @@ _SIGS = {
     "nsk_admit": ([_P, _vp, _vp, _i, _i, _vp, _vp, _i, _vp], _i),
     "nsk_advance": ([_P, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
     "nsk_finalize": ([_P, _vp, _vp, _i, _i, _vp, _vp], _i),
+    "nsk_column_pass": ([_P, _vp, _i, _vp], _i),
+    "nsk_row_pass": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp], _i),
 }
 
 _lib = None

@@ -555,6 +555,35 @@ int nsk_step(const nsk_plan* p, const float* u, float* out, const float* force, 
   return NSK_OK;
 }
 
+int nsk_column_pass(const nsk_plan* p, void* spec, int mode, void* stream) {
+  CHECK_PLAN(p);
+  if (!spec) return fail(NSK_E_INVAL, "nsk_column_pass: NULL buffer");
+  if (mode != COL_PROJECT && mode != COL_POISSON) return fail(NSK_E_INVAL, "mode must be 0 or 1, got %d", mode);
+  HIP_TRY(launch_col(p->N, mode, col_args(p, spec, nullptr, 0, REQ_ANY), p->batch, (hipStream_t)stream));
+  return NSK_OK;
+}
+
+int nsk_row_pass(const nsk_plan* p, const void* spec_in, void* spec_out, const nsk_slot* slots, int step_off,
+                 int scheme, int objective, double* jpart, int jcap, void* stream) {
+  CHECK_PLAN(p);
+  if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
+    return fail(NSK_E_INVAL, "unknown advection scheme %d", scheme);
+  if (objective != NSK_OBJ_WORK && objective != NSK_OBJ_MEAN_KE)
+    return fail(NSK_E_INVAL, "unknown objective %d", objective);
+  if (!spec_in || !spec_out || !slots) return fail(NSK_E_INVAL, "nsk_row_pass: NULL buffer");
+  if (spec_in == spec_out) return fail(NSK_E_INVAL, "nsk_row_pass: input and output spectra must differ");
+  RowArgs r = row_args(p, slots);
+  r.src_spec = (const float4*)spec_in;
+  r.dst_spec = (float4*)spec_out;
+  r.step_off = step_off;
+  r.require = REQ_ACTIVE;
+  r.part = jpart;
+  r.pcap = jcap;
+  r.objective = objective;
+  HIP_TRY(launch_row(p->N, RK_STEP_ROLL, scheme, r, p->batch, (hipStream_t)stream));
+  return NSK_OK;
+}
+
 static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
                         int scheme, int objective, double* J_out, void* spec0, void* spec1,
                         double* jpart, float* ckpt, int ckpt_every, bool use_graph, hipStream_t st) {
@@ -692,6 +721,10 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
   }
   HIP_TRY(set_step(slots, p->batch, steps, st));
   if (du0_out) {
+    // lam holds the row spectrum of lambda_0 straight from a row pass: a
+    // column pass that only rescales (1/N^2, with its own N from the column
+    // transforms) supplies the 1/N of the inverse row transform.
+    HIP_TRY(launch_col(p->N, COL_IDENT, col_args(p, lam, nullptr, 0, REQ_ANY), p->batch, st));
     RowArgs w = row_args(p, nullptr);
     w.src_spec = (const float4*)lam;
     w.dst_real = du0_out;

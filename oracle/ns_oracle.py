@@ -143,14 +143,45 @@ def inv_k2(N: int, dx: float, dtype=np.float64) -> np.ndarray:
     return out.astype(dtype)
 
 
+_FFT_WORKERS = None   # None: numpy.fft; an int: scipy.fft with that many threads
+_INV_K2_CACHE = {}
+
+
+def set_fft_workers(workers):
+    """Route the oracle's FFTs through scipy.fft with ``workers`` threads
+    (the CPU-baseline timing); None restores numpy.fft."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = workers
+
+
+def _fft2(a):
+    if _FFT_WORKERS is None:
+        return np.fft.fft2(a)
+    import scipy.fft as sf
+    return sf.fft2(a, workers=_FFT_WORKERS)
+
+
+def _ifft2(a):
+    if _FFT_WORKERS is None:
+        return np.fft.ifft2(a)
+    import scipy.fft as sf
+    return sf.ifft2(a, workers=_FFT_WORKERS)
+
+
 def poisson(rhs: np.ndarray, dx: float) -> np.ndarray:
     """p = ifft2(-fft2(rhs) / |k|^2).real with p_hat(0,0)=0, i.e. lap p = rhs
     (NS_solver.py:523-527)."""
     N = rhs.shape[0]
-    ik2 = inv_k2(N, dx, rhs.dtype)
-    p_hat = -np.fft.fft2(rhs) * ik2
+    key = (N, dx, rhs.dtype.str)
+    ik2 = _INV_K2_CACHE.get(key)
+    if ik2 is None:
+        ik2 = inv_k2(N, dx, rhs.dtype)
+        if len(_INV_K2_CACHE) > 8:
+            _INV_K2_CACHE.clear()
+        _INV_K2_CACHE[key] = ik2
+    p_hat = -_fft2(rhs) * ik2
     p_hat[0, 0] = 0.0
-    return np.fft.ifft2(p_hat).real.astype(rhs.dtype)
+    return _ifft2(p_hat).real.astype(rhs.dtype)
 
 
 # --------------------------------------------------------------------------

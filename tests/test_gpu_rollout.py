@@ -63,7 +63,9 @@ def test_golden_scheme_comparison(scheme, nu):
     s = solver(N, 1.0, nu, dt, steps, scheme)
     J, u = s.rollout(torch.zeros(2, N, N), [1.0, 0.5])
     un = u.cpu().numpy().astype(np.float64)
-    assert abs(float(J) - g[key + "_objective"]) / g[key + "_objective"] < 2e-6
+    # The state crosses four float32 FFTs per step (~1e-7 relative each
+    # step); over 164..1639 steps that random walk reaches a few 1e-6 in J.
+    assert abs(float(J) - g[key + "_objective"]) / g[key + "_objective"] < 1e-5
     assert abs(O.kinetic_energy(un, 1.0 / N) - g[key + "_kinetic_energy"]) / g[key + "_kinetic_energy"] < 1e-5
     assert abs(O.speed(un).max() - g[key + "_max_velocity"]) / g[key + "_max_velocity"] < 1e-5
 
