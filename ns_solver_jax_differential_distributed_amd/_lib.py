@@ -14,7 +14,7 @@ import threading
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnsk.so")
+LIB_PATH = os.environ.get("NSK_LIB", os.path.join(HERE, "libnsk.so"))
 
 NSK_OK = 0
 NSK_E_INVAL = -1

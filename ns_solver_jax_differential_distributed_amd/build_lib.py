@@ -44,32 +44,35 @@ def _run(cmd):
     return out
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
+def build(verbose: bool = False, jobs: int | None = None, extra_flags=(), out: str | None = None,
+          build_dir: str | None = None) -> str:
+    """Compile and link; ``extra_flags`` (e.g. -DNSK_ROW_T4096=256) and
+    ``out``/``build_dir`` allow building variants side by side."""
     hipcc = _hipcc()
-    os.makedirs(BUILD, exist_ok=True)
+    build_dir = build_dir or BUILD
+    os.makedirs(build_dir, exist_ok=True)
+    lib_path = out or LIB
     jobs = jobs or min(len(SIZES) + 1, max(1, (os.cpu_count() or 4)), 16)
-    deps = [os.path.join(CSRC, f) for f in ("nsk_fft.h", "nsk_internal.h")]
-    deps.append(os.path.join(HERE, "..", "include", "nsk.h"))
     cmds = []
     objs = []
     for n in SIZES:
-        obj = os.path.join(BUILD, f"nsk_rowcol_{n}.o")
+        obj = os.path.join(build_dir, f"nsk_rowcol_{n}.o")
         objs.append(obj)
-        cmds.append([hipcc, *_flags(), f"-DNSK_N={n}", "-c",
+        cmds.append([hipcc, *_flags(), *extra_flags, f"-DNSK_N={n}", "-c",
                      os.path.join(CSRC, "nsk_rowcol.hip"), "-o", obj])
-    api = os.path.join(BUILD, "nsk_api.o")
+    api = os.path.join(build_dir, "nsk_api.o")
     objs.append(api)
-    cmds.append([hipcc, *_flags(), "-c", os.path.join(CSRC, "nsk_api.hip"), "-o", api])
+    cmds.append([hipcc, *_flags(), *extra_flags, "-c", os.path.join(CSRC, "nsk_api.hip"), "-o", api])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        for out in ex.map(_run, cmds):
-            if verbose and out.strip():
-                print(out)
-    tmp = LIB + ".tmp"
+        for log in ex.map(_run, cmds):
+            if verbose and log.strip():
+                print(log)
+    tmp = lib_path + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib_path)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":

@@ -99,16 +99,73 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
   }
 }
 
+// Radix plan per transform length: up to three Stockham stages.
+template <int N> struct Radix;
+template <> struct Radix<4096> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 16; };
+template <> struct Radix<2048> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 8; };
+template <> struct Radix<1024> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 4; };
+template <> struct Radix<512> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 8; };
+template <> struct Radix<256> { static constexpr int S = 2, R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Radix<128> { static constexpr int S = 2, R0 = 16, R1 = 8, R2 = 1; };
+template <> struct Radix<64> { static constexpr int S = 2, R0 = 8, R1 = 8, R2 = 1; };
+template <> struct Radix<32> { static constexpr int S = 2, R0 = 8, R1 = 4, R2 = 1; };
+template <> struct Radix<16> { static constexpr int S = 1, R0 = 16, R1 = 1, R2 = 1; };
+template <> struct Radix<8> { static constexpr int S = 1, R0 = 8, R1 = 1, R2 = 1; };
+
+// Per-thread twiddles of the second and third stages for a fixed mapping
+// of (NL lines, T threads) to butterflies.  Loaded once per kernel when a
+// workgroup transforms many line sets with the same mapping; the inverse
+// transform uses the conjugates.
+template <int N, int NL, int T>
+struct TwCache {
+  using Rd = Radix<N>;
+  static constexpr int P1 = Rd::R0, P2 = Rd::R0 * Rd::R1;
+  static constexpr int B1 = Rd::S > 1 ? (NL * (N / Rd::R1) + T - 1) / T : 1;
+  static constexpr int B2 = Rd::S > 2 ? (NL * (N / Rd::R2) + T - 1) / T : 1;
+  static constexpr int C1 = Rd::S > 1 ? Rd::R1 - 1 : 1;
+  static constexpr int C2 = Rd::S > 2 ? Rd::R2 - 1 : 1;
+  float2 w1[B1][C1];
+  float2 w2[B2][C2];
+
+  __device__ __forceinline__ void load(int tid) {
+    if constexpr (Rd::S > 1) {
+      constexpr int NB = N / Rd::R1, TOT = NL * NB;
+#pragma unroll
+      for (int q = 0; q < B1; ++q) {
+        const int b = tid + q * T;
+        const int k = (b % NB) & (P1 - 1);
+#pragma unroll
+        for (int j = 1; j <= C1; ++j)
+          w1[q][j - 1] = (b < TOT) ? g_tw[(j * k) * (TW_N / (P1 * Rd::R1))] : make_float2(1.f, 0.f);
+      }
+    }
+    if constexpr (Rd::S > 2) {
+      constexpr int NB = N / Rd::R2, TOT = NL * NB;
+#pragma unroll
+      for (int q = 0; q < B2; ++q) {
+        const int b = tid + q * T;
+        const int k = (b % NB) & (P2 - 1);
+#pragma unroll
+        for (int j = 1; j <= C2; ++j)
+          w2[q][j - 1] = (b < TOT) ? g_tw[(j * k) * (TW_N / (P2 * Rd::R2))] : make_float2(1.f, 0.f);
+      }
+    }
+  }
+};
+
 // One Stockham stage of radix R on NL lines of length N held in LDS.
 // Line l lives at buf + ((first + l) % MOD) * NP.  P = product of the radices
 // of the earlier stages.  T threads cooperate; every thread must call this.
-template <int N, int R, int P, int DIR, int NL, int T, int MOD>
-__device__ __forceinline__ void fft_stage(float2* buf, int first, int tid) {
+// tw (optional): this thread's twiddles [BPT][R-1] for the stage.
+template <int N, int R, int P, int DIR, int NL, int T, int MOD, int BPT_TW = 0, int CT = 1>
+__device__ __forceinline__ void fft_stage(float2* buf, int first, int tid,
+                                          const float2 (*tw)[CT] = nullptr) {
   constexpr int NB = N / R;
   constexpr int TOT = NL * NB;
   constexpr int BPT = (TOT + T - 1) / T;
   constexpr int NP = LineLen<N>::NP;
   static_assert(BPT * R <= 48, "too many points per thread");
+  static_assert(BPT_TW == 0 || (BPT_TW == BPT && CT == R - 1), "twiddle cache shape");
   float2 x[BPT][R];
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
@@ -122,7 +179,11 @@ __device__ __forceinline__ void fft_stage(float2* buf, int first, int tid) {
         const int k = i & (P - 1);
 #pragma unroll
         for (int j = 1; j < R; ++j) {
-          float2 w = g_tw[(j * k) * (TW_N / (P * R))];
+          float2 w;
+          if constexpr (BPT_TW > 0)
+            w = tw[q][j - 1];
+          else
+            w = g_tw[(j * k) * (TW_N / (P * R))];
           if (DIR > 0) w.y = -w.y;
           x[q][j] = cmul(x[q][j], w);
         }
@@ -149,41 +210,23 @@ __device__ __forceinline__ void fft_stage(float2* buf, int first, int tid) {
 // Full unnormalised transform of NL lines: DIR = -1 forward, +1 inverse.
 template <int N, int DIR, int NL, int T, int MOD>
 __device__ __forceinline__ void fft_lines(float2* buf, int first, int tid) {
-  if constexpr (N == 4096) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 16, 16, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 16, 256, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 2048) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 16, 16, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 8, 256, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 1024) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 16, 16, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 4, 256, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 512) {
-    fft_stage<N, 8, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 8, 8, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 8, 64, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 256) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 16, 16, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 128) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 8, 16, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 64) {
-    fft_stage<N, 8, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 8, 8, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 32) {
-    fft_stage<N, 8, 1, DIR, NL, T, MOD>(buf, first, tid);
-    fft_stage<N, 4, 8, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 16) {
-    fft_stage<N, 16, 1, DIR, NL, T, MOD>(buf, first, tid);
-  } else if constexpr (N == 8) {
-    fft_stage<N, 8, 1, DIR, NL, T, MOD>(buf, first, tid);
-  } else {
-    static_assert(N == 8, "unsupported N");
-  }
+  using Rd = Radix<N>;
+  fft_stage<N, Rd::R0, 1, DIR, NL, T, MOD>(buf, first, tid);
+  if constexpr (Rd::S > 1) fft_stage<N, Rd::R1, Rd::R0, DIR, NL, T, MOD>(buf, first, tid);
+  if constexpr (Rd::S > 2) fft_stage<N, Rd::R2, Rd::R0 * Rd::R1, DIR, NL, T, MOD>(buf, first, tid);
+}
+
+// Same with this thread's twiddles from a TwCache built for (NL, T).
+template <int N, int DIR, int NL, int T, int MOD>
+__device__ __forceinline__ void fft_lines_tc(float2* buf, int first, int tid,
+                                             const TwCache<N, NL, T>& tc) {
+  using Rd = Radix<N>;
+  using TC = TwCache<N, NL, T>;
+  fft_stage<N, Rd::R0, 1, DIR, NL, T, MOD>(buf, first, tid);
+  if constexpr (Rd::S > 1)
+    fft_stage<N, Rd::R1, Rd::R0, DIR, NL, T, MOD, TC::B1, TC::C1>(buf, first, tid, tc.w1);
+  if constexpr (Rd::S > 2)
+    fft_stage<N, Rd::R2, Rd::R0 * Rd::R1, DIR, NL, T, MOD, TC::B2, TC::C2>(buf, first, tid, tc.w2);
 }
 
 }  // namespace nsk

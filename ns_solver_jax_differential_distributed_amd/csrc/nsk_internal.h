@@ -70,44 +70,70 @@ struct StencilArgs {
 };
 
 // Row-pass geometry for grid size N: G rows per iteration, ITER iterations
-// per workgroup, 256 threads.
+// per workgroup (a band of G*ITER rows), T threads.
+#ifndef NSK_ROW_T4096
+#define NSK_ROW_T4096 256
+#endif
+#ifndef NSK_ROW_TWR
+#define NSK_ROW_TWR 0
+#endif
+#ifndef NSK_COL_W4096
+#define NSK_COL_W4096 2
+#endif
 template <int N>
 struct RowCfg;
-template <> struct RowCfg<4096> { static constexpr int G = 1, ITER = 16; };
-template <> struct RowCfg<2048> { static constexpr int G = 2, ITER = 4; };
-template <> struct RowCfg<1024> { static constexpr int G = 4, ITER = 2; };
-template <> struct RowCfg<512> { static constexpr int G = 8, ITER = 1; };
-template <> struct RowCfg<256> { static constexpr int G = 16, ITER = 1; };
-template <> struct RowCfg<128> { static constexpr int G = 32, ITER = 1; };
-template <> struct RowCfg<64> { static constexpr int G = 64, ITER = 1; };
-template <> struct RowCfg<32> { static constexpr int G = 32, ITER = 1; };
-template <> struct RowCfg<16> { static constexpr int G = 16, ITER = 1; };
-template <> struct RowCfg<8> { static constexpr int G = 8, ITER = 1; };
+template <> struct RowCfg<4096> { static constexpr int G = 2, ITER = 8, T = NSK_ROW_T4096; };
+template <> struct RowCfg<2048> { static constexpr int G = 2, ITER = 4, T = 256; };
+template <> struct RowCfg<1024> { static constexpr int G = 4, ITER = 2, T = 256; };
+template <> struct RowCfg<512> { static constexpr int G = 8, ITER = 1, T = 512; };
+template <> struct RowCfg<256> { static constexpr int G = 16, ITER = 1, T = 256; };
+template <> struct RowCfg<128> { static constexpr int G = 32, ITER = 1, T = 256; };
+template <> struct RowCfg<64> { static constexpr int G = 64, ITER = 1, T = 512; };
+template <> struct RowCfg<32> { static constexpr int G = 32, ITER = 1, T = 256; };
+template <> struct RowCfg<16> { static constexpr int G = 16, ITER = 1, T = 256; };
+template <> struct RowCfg<8> { static constexpr int G = 8, ITER = 1, T = 256; };
 
-// Column-pass geometry: W spectral columns (two fields each) per workgroup.
+// Column-pass geometry: W spectral columns (two fields each) per block, T
+// threads, NWG workgroups per slot (each loops over blocks NWG apart).
 template <int N>
 struct ColCfg;
-template <> struct ColCfg<4096> { static constexpr int W = 2, T = 512; };
-template <> struct ColCfg<2048> { static constexpr int W = 2, T = 512; };
-template <> struct ColCfg<1024> { static constexpr int W = 2, T = 256; };
-template <> struct ColCfg<512> { static constexpr int W = 2, T = 256; };
-template <> struct ColCfg<256> { static constexpr int W = 16, T = 512; };
-template <> struct ColCfg<128> { static constexpr int W = 32, T = 512; };
-template <> struct ColCfg<64> { static constexpr int W = 32, T = 512; };
-template <> struct ColCfg<32> { static constexpr int W = 16, T = 256; };
-template <> struct ColCfg<16> { static constexpr int W = 8, T = 256; };
-template <> struct ColCfg<8> { static constexpr int W = 4, T = 256; };
+#ifndef NSK_COL_T4096
+#define NSK_COL_T4096 512
+#endif
+#ifndef NSK_COL_NWG4096
+#define NSK_COL_NWG4096 1024
+#endif
+#ifndef NSK_COL_WPE
+#define NSK_COL_WPE 0
+#endif
+template <> struct ColCfg<4096> {
+  static constexpr int W = NSK_COL_W4096, T = NSK_COL_T4096, NWG = NSK_COL_NWG4096;
+};
+template <> struct ColCfg<2048> { static constexpr int W = 1, T = 256, NWG = 512; };
+template <> struct ColCfg<1024> { static constexpr int W = 2, T = 256, NWG = 256; };
+template <> struct ColCfg<512> { static constexpr int W = 2, T = 256, NWG = 128; };
+template <> struct ColCfg<256> { static constexpr int W = 16, T = 512, NWG = 8; };
+template <> struct ColCfg<128> { static constexpr int W = 32, T = 512, NWG = 2; };
+template <> struct ColCfg<64> { static constexpr int W = 32, T = 512, NWG = 1; };
+template <> struct ColCfg<32> { static constexpr int W = 16, T = 256, NWG = 1; };
+template <> struct ColCfg<16> { static constexpr int W = 8, T = 256, NWG = 1; };
+template <> struct ColCfg<8> { static constexpr int W = 4, T = 256, NWG = 1; };
 
-constexpr int ROW_T = 256;
+template <int N>
+constexpr int row_bands_c() { return N / (RowCfg<N>::G * RowCfg<N>::ITER); }
 
 inline int row_bands(int N) {
   switch (N) {
-    case 4096: return 4096 / (RowCfg<4096>::G * RowCfg<4096>::ITER);
-    case 2048: return 2048 / (RowCfg<2048>::G * RowCfg<2048>::ITER);
-    case 1024: return 1024 / (RowCfg<1024>::G * RowCfg<1024>::ITER);
-    case 512: return 512 / RowCfg<512>::G;
-    case 256: return 256 / RowCfg<256>::G;
-    case 128: return 128 / RowCfg<128>::G;
+    case 4096: return row_bands_c<4096>();
+    case 2048: return row_bands_c<2048>();
+    case 1024: return row_bands_c<1024>();
+    case 512: return row_bands_c<512>();
+    case 256: return row_bands_c<256>();
+    case 128: return row_bands_c<128>();
+    case 64: return row_bands_c<64>();
+    case 32: return row_bands_c<32>();
+    case 16: return row_bands_c<16>();
+    case 8: return row_bands_c<8>();
     default: return 1;
   }
 }

@@ -192,9 +192,84 @@ __device__ __forceinline__ void join_pair(float2* L, int c, float4 h) {
   }
 }
 
+// Row-pass helpers: G rows move between global memory and the LDS ring
+// through registers, so the loads of the next rows can be issued before the
+// current rows are processed.
+template <int N, int G, int T>
+struct SpecRows {
+  static constexpr int HN = N / 2, PF = (G * HN + T - 1) / T;
+  float4 v[PF];
+  __device__ __forceinline__ void fetch(const float4* src, int rfirst, int tid) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = tid + q * T;
+      if ((G * HN) % T == 0 || e < G * HN) {
+        const int l = e / HN, c = e % HN;
+        v[q] = src[(size_t)((rfirst + l) & (N - 1)) * HN + c];
+      }
+    }
+  }
+  // expand into the row FFT input Z of (a + i b) in ring slots
+  template <int RING>
+  __device__ __forceinline__ void commit(float2* ring, int sfirst, int tid) const {
+    constexpr int NP = LineLen<N>::NP;
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = tid + q * T;
+      if ((G * HN) % T == 0 || e < G * HN) {
+        const int l = e / HN, c = e % HN;
+        join_pair<N>(ring + ((sfirst + l) % RING) * NP, c, v[q]);
+      }
+    }
+  }
+};
+
+template <int N, int G, int T>
+struct RealRows {
+  static constexpr int Q = N / 4, PF = (G * Q + T - 1) / T;
+  float4 vx[PF], vy[PF];
+  __device__ __forceinline__ void fetch(const float* src, bool two, int rfirst, int tid) {
+    const size_t NN = (size_t)N * N;
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = tid + q * T;
+      if ((G * Q) % T == 0 || e < G * Q) {
+        const int l = e / Q, y4 = e % Q;
+        const size_t off = (size_t)((rfirst + l) & (N - 1)) * N + 4 * y4;
+        vx[q] = *reinterpret_cast<const float4*>(src + off);
+        vy[q] = two ? *reinterpret_cast<const float4*>(src + NN + off) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+  }
+  template <int RING>
+  __device__ __forceinline__ void commit(float2* ring, int sfirst, int tid) const {
+    constexpr int NP = LineLen<N>::NP;
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = tid + q * T;
+      if ((G * Q) % T == 0 || e < G * Q) {
+        const int l = e / Q, y = 4 * (e % Q);
+        float2* L = ring + ((sfirst + l) % RING) * NP;
+        L[pad(y)] = make_float2(vx[q].x, vy[q].x);
+        L[pad(y + 1)] = make_float2(vx[q].y, vy[q].y);
+        L[pad(y + 2)] = make_float2(vx[q].z, vy[q].z);
+        L[pad(y + 3)] = make_float2(vx[q].w, vy[q].w);
+      }
+    }
+  }
+};
+
+// The row pass.  A workgroup owns a band of G*ITER consecutive rows and
+// marches through it G rows at a time.  The LDS ring holds RING = G + 2*HALO
+// rows of real pairs (u_x, u_y): rows xb-HALO .. xb+G-1+HALO for the rows
+// xb .. xb+G-1 being processed.  Outputs of the predictor go back into the
+// slots of rows xb-1 .. xb+G-2, which no later row needs; after they have
+// been transformed and stored, the next rows (prefetched into registers at
+// the top of the iteration) take the same slots.  Row r sits in slot
+// (r - x0 + HALO) % RING.
 template <int N, int KIND, int SCHEME>
-__global__ void __launch_bounds__(ROW_T) k_row(RowArgs a) {
-  constexpr int G = RowCfg<N>::G, ITER = RowCfg<N>::ITER, T = ROW_T;
+__global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
+  constexpr int G = RowCfg<N>::G, ITER = RowCfg<N>::ITER, T = RowCfg<N>::T;
   constexpr int NP = LineLen<N>::NP, HN = N / 2;
   constexpr int SRC = (KIND == RK_REAL_TO_SPEC || KIND == RK_STEP_FROM_REAL || KIND == RK_STEP_FORCE)
                           ? SRC_REAL : SRC_SPEC;
@@ -204,9 +279,12 @@ __global__ void __launch_bounds__(ROW_T) k_row(RowArgs a) {
   constexpr bool ROLL = (KIND == RK_STEP_ROLL);
   constexpr int HALO = (OP == OP_COPY) ? 0 : 1;
   constexpr int RING = G + 2 * HALO;
-  constexpr int NOUT = (OP == OP_COPY) ? 0 : G;
+  constexpr bool TWR = NSK_ROW_TWR && ITER > 1;  // twiddles reused across iterations
   constexpr int GD = (OP == OP_ADJ) ? G : 1;
-  __shared__ float2 lds[(RING + NOUT) * NP];
+  constexpr int PH = (G == 2) ? 2 : 1;      // stencil phases
+  constexpr int GP = G / PH;                // rows per phase
+  constexpr int RES = (GP * N + T - 1) / T;
+  __shared__ float2 lds[RING * NP];
   __shared__ float s_f[G];
   __shared__ double s_dA[GD], s_dB[GD];
   __shared__ double s_red[T / 64];
@@ -239,6 +317,7 @@ __global__ void __launch_bounds__(ROW_T) k_row(RowArgs a) {
   const float4* srcS = (SRC == SRC_SPEC) ? a.src_spec + (size_t)b * N * HN : nullptr;
   float4* dstS = a.dst_spec ? a.dst_spec + (size_t)b * N * HN : nullptr;
   float* dstR = a.dst_real ? a.dst_real + (long long)b * a.dst_bstride : nullptr;
+  const bool two = a.src_nc > 1;
 
   float* st = nullptr;
   if (OP == OP_STEP && a.ustore != nullptr && !final_mode) {
@@ -251,75 +330,90 @@ __global__ void __launch_bounds__(ROW_T) k_row(RowArgs a) {
 
   const int x0 = band * G * ITER;
   float2* ring = lds;
-  float2* out = lds + RING * NP;
   double acc0 = 0.0, acc1 = 0.0;
   const double dx2 = p.dx * p.dx;
+
+  TwCache<N, G, T> tc;
+  if constexpr (TWR) tc.load(tid);
+
+  // ---- prologue: rows x0-HALO .. x0+G-1+HALO into slots 0 .. RING-1
+  if constexpr (SRC == SRC_SPEC) {
+    for (int e = tid; e < RING * HN; e += T) {
+      const int l = e / HN, c = e % HN;
+      join_pair<N>(ring + l * NP, c, srcS[(size_t)((x0 - HALO + l) & (N - 1)) * HN + c]);
+    }
+    __syncthreads();
+    if constexpr (HALO == 0 || G == 2) {
+#pragma unroll
+      for (int part = 0; part < RING / G; ++part) {
+        if constexpr (TWR)
+          fft_lines_tc<N, +1, G, T, RING>(ring, part * G, tid, tc);
+        else
+          fft_lines<N, +1, G, T, RING>(ring, part * G, tid);
+      }
+    } else {
+      fft_lines<N, +1, RING, T, RING>(ring, 0, tid);
+    }
+  } else {
+    const float* s1 = srcR + NN;
+    for (int e = tid; e < RING * (N / 4); e += T) {
+      const int l = e / (N / 4), y = 4 * (e % (N / 4));
+      const size_t off = (size_t)((x0 - HALO + l) & (N - 1)) * N + y;
+      const float4 vx = *reinterpret_cast<const float4*>(srcR + off);
+      const float4 vy = two ? *reinterpret_cast<const float4*>(s1 + off) : make_float4(0.f, 0.f, 0.f, 0.f);
+      float2* L = ring + l * NP;
+      L[pad(y)] = make_float2(vx.x, vy.x);
+      L[pad(y + 1)] = make_float2(vx.y, vy.y);
+      L[pad(y + 2)] = make_float2(vx.z, vy.z);
+      L[pad(y + 3)] = make_float2(vx.w, vy.w);
+    }
+    __syncthreads();
+  }
 
 #pragma unroll 1
   for (int it = 0; it < ITER; ++it) {
     const int xb = x0 + it * G;
-    const int nload = (it == 0) ? RING : G;
-    const int rfirst = (it == 0) ? xb - HALO : xb + HALO;
-    const int sfirst = (rfirst - (x0 - HALO)) % RING;
+    const bool has_next = it + 1 < ITER;
+    const int s0 = (xb - x0) % RING;           // first slot freed by this iteration
+    const int rnext = xb + G + HALO;           // first row loaded for the next one
 
-    // ---- rows into the ring (real pairs; spectra are inverse transformed)
-    if constexpr (SRC == SRC_SPEC) {
-      for (int e = tid; e < nload * HN; e += T) {
-        const int l = e / HN, c = e % HN;
-        const int r = (rfirst + l) & (N - 1);
-        join_pair<N>(ring + ((sfirst + l) % RING) * NP, c, srcS[(size_t)r * HN + c]);
-      }
-      __syncthreads();
-      if (it == 0 && HALO > 0) {
-        // halo rows x0-1 and x0+G sit in slots 0 and G+1 (ring order)
-        fft_lines<N, +1, 2, T, RING>(ring, G + 1, tid);
-        fft_lines<N, +1, G, T, RING>(ring, 1, tid);
-      } else {
-        fft_lines<N, +1, G, T, RING>(ring, sfirst, tid);
-      }
-    } else {
-      const float* s1 = srcR + NN;
-      const bool two = a.src_nc > 1;
-      for (int e = tid; e < nload * N; e += T) {
-        const int l = e / N, y = e % N;
-        const int r = (rfirst + l) & (N - 1);
-        const float vx = srcR[(size_t)r * N + y];
-        const float vy = two ? s1[(size_t)r * N + y] : 0.0f;
-        ring[((sfirst + l) % RING) * NP + pad(y)] = make_float2(vx, vy);
-      }
-      __syncthreads();
+    // ---- prefetch the next rows into registers
+    SpecRows<N, G, T> pfs;
+    RealRows<N, G, T> pfr;
+    if (has_next) {
+      if constexpr (SRC == SRC_SPEC)
+        pfs.fetch(srcS, rnext, tid);
+      else
+        pfr.fetch(srcR, two, rnext, tid);
     }
 
     if constexpr (OP == OP_COPY) {
       if constexpr (SRC == SRC_SPEC) {
-        const bool two = a.dst_nc > 1;
+        const bool two_out = a.dst_nc > 1;
         for (int e = tid; e < G * N; e += T) {
           const int g = e / N, y = e % N;
-          const float2 v = ring[g * NP + pad(y)];
+          const float2 v = ring[((s0 + g) % RING) * NP + pad(y)];
           dstR[(size_t)(xb + g) * N + y] = v.x;
-          if (two) dstR[NN + (size_t)(xb + g) * N + y] = v.y;
+          if (two_out) dstR[NN + (size_t)(xb + g) * N + y] = v.y;
         }
       } else {
-        fft_lines<N, -1, G, T, G>(ring, 0, tid);
+        if constexpr (TWR)
+          fft_lines_tc<N, -1, G, T, RING>(ring, s0, tid, tc);
+        else
+          fft_lines<N, -1, G, T, RING>(ring, s0, tid);
         for (int e = tid; e < G * HN; e += T) {
           const int g = e / HN, c = e % HN;
-          dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(ring + g * NP, c);
+          dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(ring + ((s0 + g) % RING) * NP, c);
         }
       }
-      __syncthreads();
-      continue;
+    } else if (ROLL && final_mode) {
+      for (int e = tid; e < G * N; e += T) {
+        const int g = e / N, y = e % N;
+        const float2 v = ring[((s0 + 1 + g) % RING) * NP + pad(y)];
+        dstR[(size_t)(xb + g) * N + y] = v.x;
+        dstR[NN + (size_t)(xb + g) * N + y] = v.y;
+      }
     } else {
-      if (ROLL && final_mode) {
-        for (int e = tid; e < G * N; e += T) {
-          const int g = e / N, y = e % N;
-          const int x = xb + g;
-          const float2 v = ring[((x - (x0 - 1)) % RING) * NP + pad(y)];
-          dstR[(size_t)x * N + y] = v.x;
-          dstR[NN + (size_t)x * N + y] = v.y;
-        }
-        __syncthreads();
-        continue;
-      }
       // ---- per-row forcing (f_x depends on x only, f_y = 0)
       if (tid < G) {
         const int x = xb + tid;
@@ -334,69 +428,104 @@ __global__ void __launch_bounds__(ROW_T) k_row(RowArgs a) {
       }
       __syncthreads();
 
-      // ---- stencil
+      // ---- stencil, PH phases of GP rows: outputs are held in registers
+      // until the phase's reads are done, then written into the slots of
+      // rows xb-1+g, which later rows no longer read.
       const float* F = FFIELD ? a.force + (long long)b * a.force_bstride : nullptr;
       const float scale_ke = (a.objective == NSK_OBJ_MEAN_KE)
                                  ? (float)(1.0 / ((double)sl.nsteps * (double)NN)) : 0.0f;
       const float jdir = (float)((double)dt * dx2);
-      for (int e = tid; e < G * N; e += T) {
-        const int g = e / N, y = e % N;
-        const int x = xb + g;
-        const int rc = x - (x0 - 1);
-        const float2* Lc = ring + (rc % RING) * NP;
-        const float2* Lm = ring + ((rc - 1) % RING) * NP;
-        const float2* Lp = ring + ((rc + 1) % RING) * NP;
-        const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
-        const float2 c = Lc[pad(y)], l = Lm[pad(y)], r = Lp[pad(y)];
-        const float2 d = Lc[pad(ym)], u = Lc[pad(yp)];
-        float2 res;
-        if constexpr (OP == OP_STEP) {
-          float fx = s_f[g], fy = 0.0f;
-          if constexpr (FFIELD) {
-            fx = F[(size_t)x * N + y];
-            fy = F[NN + (size_t)x * N + y];
-          } else {
-            if (a.objective == NSK_OBJ_WORK)
-              acc0 += (double)fx * (double)c.x;
-            else
-              acc0 += 0.5 * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
+#pragma unroll
+      for (int ph = 0; ph < PH; ++ph) {
+        float2 res[RES];
+#pragma unroll
+        for (int q = 0; q < RES; ++q) {
+          const int e = tid + q * T;
+          if ((GP * N) % T == 0 || e < GP * N) {
+            const int g = ph * GP + e / N, y = e % N;
+            const int x = xb + g;
+            const float2* Lm = ring + ((s0 + g) % RING) * NP;      // row x-1
+            const float2* Lc = ring + ((s0 + g + 1) % RING) * NP;  // row x
+            const float2* Lp = ring + ((s0 + g + 2) % RING) * NP;  // row x+1
+            const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
+            const float2 c = Lc[pad(y)], l = Lm[pad(y)], r = Lp[pad(y)];
+            const float2 d = Lc[pad(ym)], u = Lc[pad(yp)];
+            if constexpr (OP == OP_STEP) {
+              float fx = s_f[g], fy = 0.0f;
+              if constexpr (FFIELD) {
+                fx = F[(size_t)x * N + y];
+                fy = F[NN + (size_t)x * N + y];
+              } else {
+                if (a.objective == NSK_OBJ_WORK)
+                  acc0 += (double)fx * (double)c.x;
+                else
+                  acc0 += 0.5 * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
+              }
+              if (st) {
+                st[(size_t)x * N + y] = c.x;
+                st[NN + (size_t)x * N + y] = c.y;
+              }
+              res[q] = predictor<SCHEME>(c, l, r, d, u, fx, fy, nu, dt, p);
+            } else {
+              const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+              const float* U0 = us;
+              const float* U1 = us + NN;
+              const float2 Uc = make_float2(U0[(size_t)x * N + y], U1[(size_t)x * N + y]);
+              const float2 Ul = make_float2(U0[(size_t)xm * N + y], U1[(size_t)xm * N + y]);
+              const float2 Ur = make_float2(U0[(size_t)xp * N + y], U1[(size_t)xp * N + y]);
+              const float2 Ud = make_float2(U0[(size_t)x * N + ym], U1[(size_t)x * N + ym]);
+              const float2 Uu = make_float2(U0[(size_t)x * N + yp], U1[(size_t)x * N + yp]);
+              float2 v = predictor_vjp<SCHEME>(c, l, r, d, u, Uc, Ul, Ur, Ud, Uu, nu, dt, p);
+              double grow;
+              if (a.objective == NSK_OBJ_WORK) {
+                v.x += jdir * s_f[g];
+                grow = (double)dt * dx2 * (double)Uc.x + (double)dt * (double)c.x;
+              } else {
+                v.x += scale_ke * Uc.x;
+                v.y += scale_ke * Uc.y;
+                grow = (double)dt * (double)c.x;
+              }
+              acc0 += grow * s_dA[g];
+              acc1 += grow * s_dB[g];
+              res[q] = v;
+            }
           }
-          if (st) {
-            st[(size_t)x * N + y] = c.x;
-            st[NN + (size_t)x * N + y] = c.y;
-          }
-          res = predictor<SCHEME>(c, l, r, d, u, fx, fy, nu, dt, p);
-        } else {
-          const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
-          const float* U0 = us;
-          const float* U1 = us + NN;
-          const float2 Uc = make_float2(U0[(size_t)x * N + y], U1[(size_t)x * N + y]);
-          const float2 Ul = make_float2(U0[(size_t)xm * N + y], U1[(size_t)xm * N + y]);
-          const float2 Ur = make_float2(U0[(size_t)xp * N + y], U1[(size_t)xp * N + y]);
-          const float2 Ud = make_float2(U0[(size_t)x * N + ym], U1[(size_t)x * N + ym]);
-          const float2 Uu = make_float2(U0[(size_t)x * N + yp], U1[(size_t)x * N + yp]);
-          res = predictor_vjp<SCHEME>(c, l, r, d, u, Uc, Ul, Ur, Ud, Uu, nu, dt, p);
-          double grow;
-          if (a.objective == NSK_OBJ_WORK) {
-            res.x += jdir * s_f[g];
-            grow = (double)dt * dx2 * (double)Uc.x + (double)dt * (double)c.x;
-          } else {
-            res.x += scale_ke * Uc.x;
-            res.y += scale_ke * Uc.y;
-            grow = (double)dt * (double)c.x;
-          }
-          acc0 += grow * s_dA[g];
-          acc1 += grow * s_dB[g];
         }
-        out[g * NP + pad(y)] = res;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RES; ++q) {
+          const int e = tid + q * T;
+          if ((GP * N) % T == 0 || e < GP * N) {
+            const int g = ph * GP + e / N, y = e % N;
+            ring[((s0 + g) % RING) * NP + pad(y)] = res[q];
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      fft_lines<N, -1, G, T, G>(out, 0, tid);
+      if constexpr (TWR)
+        fft_lines_tc<N, -1, G, T, RING>(ring, s0, tid, tc);
+      else
+        fft_lines<N, -1, G, T, RING>(ring, s0, tid);
       for (int e = tid; e < G * HN; e += T) {
         const int g = e / HN, c = e % HN;
-        dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(out + g * NP, c);
+        dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(ring + ((s0 + g) % RING) * NP, c);
       }
+    }
+    __syncthreads();
+
+    // ---- next rows into the freed slots
+    if (has_next) {
+      if constexpr (SRC == SRC_SPEC)
+        pfs.template commit<RING>(ring, s0, tid);
+      else
+        pfr.template commit<RING>(ring, s0, tid);
       __syncthreads();
+      if constexpr (SRC == SRC_SPEC) {
+        if constexpr (TWR)
+          fft_lines_tc<N, +1, G, T, RING>(ring, s0, tid, tc);
+        else
+          fft_lines<N, +1, G, T, RING>(ring, s0, tid);
+      }
     }
   }
 
@@ -449,66 +578,109 @@ __device__ __forceinline__ void apply_mode(float2& X, float2& Y, int kx, int ky,
   }
 }
 
+// The column pass.  Each workgroup handles blocks of W spectral columns
+// (both fields: 2W lines of N points), blocks NWG apart, and prefetches the
+// next block into registers while it transforms the current one.
+#if NSK_COL_WPE > 0
+#define NSK_COL_ATTR __attribute__((amdgpu_waves_per_eu(NSK_COL_WPE)))
+#else
+#define NSK_COL_ATTR
+#endif
 template <int N, int MODE>
-__global__ void __launch_bounds__(ColCfg<N>::T) k_col(ColArgs a) {
+__global__ void __launch_bounds__(ColCfg<N>::T) NSK_COL_ATTR k_col(ColArgs a) {
   constexpr int W = ColCfg<N>::W, T = ColCfg<N>::T, NL = 2 * W;
-  constexpr int NP = LineLen<N>::NP, HN = N / 2;
+  constexpr int NP = LineLen<N>::NP, HN = N / 2, NBLK = HN / W;
+  constexpr int PF = (N * W + T - 1) / T;
   __shared__ float2 lds[NL * NP];
   const int tid = threadIdx.x, b = blockIdx.y;
-  const int kb = xcd_remap(blockIdx.x, gridDim.x);
+  const int nwg = gridDim.x;
+  const int w0 = xcd_remap(blockIdx.x, nwg);
   if (a.require == REQ_ACTIVE) {
     const nsk_slot sl = a.slots[b];
     const int n = sl.step + a.step_off;
     if (n < 0 || n >= sl.nsteps) return;
   }
   float4* H = a.spec + (size_t)b * N * HN;
-  for (int e = tid; e < N * W; e += T) {
-    const int x = e / W, kw = e % W;
-    const float4 h = H[(size_t)x * HN + kb * W + kw];
-    lds[(2 * kw) * NP + pad(x)] = make_float2(h.x, h.y);
-    lds[(2 * kw + 1) * NP + pad(x)] = make_float2(h.z, h.w);
-  }
-  __syncthreads();
-  fft_lines<N, -1, NL, T, NL>(lds, 0, tid);
-  const float scale = 1.0f / ((float)N * (float)N);
   const nsk_plan& p = a.plan;
-  for (int e = tid; e < W * N; e += T) {
-    const int kw = e / N, kx = e % N;
-    const int ky = kb * W + kw;
-    float2* LX = lds + (2 * kw) * NP;
-    float2* LY = LX + NP;
-    if (ky == 0) {
-      // column 0 carries ky = 0 (real part of the row data) and ky = N/2
-      // (imaginary part); separate them, treat each, recombine.
-      if (kx > HN) continue;
-      const int kx2 = (N - kx) & (N - 1);
-      const float2 x1 = LX[pad(kx)], x2 = LX[pad(kx2)];
-      const float2 y1 = LY[pad(kx)], y2 = LY[pad(kx2)];
-      float2 AX = make_float2(0.5f * (x1.x + x2.x), 0.5f * (x1.y - x2.y));
-      float2 BX = make_float2(0.5f * (x1.y + x2.y), 0.5f * (x2.x - x1.x));
-      float2 AY = make_float2(0.5f * (y1.x + y2.x), 0.5f * (y1.y - y2.y));
-      float2 BY = make_float2(0.5f * (y1.y + y2.y), 0.5f * (y2.x - y1.x));
-      apply_mode<N, MODE>(AX, AY, kx, 0, p);
-      apply_mode<N, MODE>(BX, BY, kx, HN, p);
-      LX[pad(kx)] = make_float2((AX.x - BX.y) * scale, (AX.y + BX.x) * scale);
-      LY[pad(kx)] = make_float2((AY.x - BY.y) * scale, (AY.y + BY.x) * scale);
-      if (kx2 != kx) {
-        LX[pad(kx2)] = make_float2((AX.x + BX.y) * scale, (BX.x - AX.y) * scale);
-        LY[pad(kx2)] = make_float2((AY.x + BY.y) * scale, (BY.x - AY.y) * scale);
+  const float scale = 1.0f / ((float)N * (float)N);
+
+  float4 pf[PF];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int e = tid + q * T;
+      if ((N * W) % T == 0 || e < N * W) {
+        const int x = e / W, kw = e % W;
+        pf[q] = H[(size_t)x * HN + kb * W + kw];
+      }
+    }
+  };
+  // one block per workgroup: load straight into LDS; otherwise stage the
+  // next block in registers while the current one is transformed
+  constexpr bool PERSIST = ColCfg<N>::NWG < NBLK;
+  if (PERSIST && w0 < NBLK) fetch(w0);
+#pragma unroll 1
+  for (int kb = w0; kb < NBLK; kb += nwg) {
+    if constexpr (PERSIST) {
+#pragma unroll
+      for (int q = 0; q < PF; ++q) {
+        const int e = tid + q * T;
+        if ((N * W) % T == 0 || e < N * W) {
+          const int x = e / W, kw = e % W;
+          lds[(2 * kw) * NP + pad(x)] = make_float2(pf[q].x, pf[q].y);
+          lds[(2 * kw + 1) * NP + pad(x)] = make_float2(pf[q].z, pf[q].w);
+        }
       }
     } else {
-      float2 X = LX[pad(kx)], Y = LY[pad(kx)];
-      apply_mode<N, MODE>(X, Y, kx, ky, p);
-      LX[pad(kx)] = make_float2(X.x * scale, X.y * scale);
-      LY[pad(kx)] = make_float2(Y.x * scale, Y.y * scale);
+      for (int e = tid; e < N * W; e += T) {
+        const int x = e / W, kw = e % W;
+        const float4 h = H[(size_t)x * HN + kb * W + kw];
+        lds[(2 * kw) * NP + pad(x)] = make_float2(h.x, h.y);
+        lds[(2 * kw + 1) * NP + pad(x)] = make_float2(h.z, h.w);
+      }
     }
-  }
-  __syncthreads();
-  fft_lines<N, +1, NL, T, NL>(lds, 0, tid);
-  for (int e = tid; e < N * W; e += T) {
-    const int x = e / W, kw = e % W;
-    const float2 X = lds[(2 * kw) * NP + pad(x)], Y = lds[(2 * kw + 1) * NP + pad(x)];
-    H[(size_t)x * HN + kb * W + kw] = make_float4(X.x, X.y, Y.x, Y.y);
+    __syncthreads();
+    if (PERSIST && kb + nwg < NBLK) fetch(kb + nwg);
+    fft_lines<N, -1, NL, T, NL>(lds, 0, tid);
+    for (int e = tid; e < W * N; e += T) {
+      const int kw = e / N, kx = e % N;
+      const int ky = kb * W + kw;
+      float2* LX = lds + (2 * kw) * NP;
+      float2* LY = LX + NP;
+      if (ky == 0) {
+        // column 0 carries ky = 0 (real part of the row data) and ky = N/2
+        // (imaginary part); separate them, treat each, recombine.
+        if (kx > HN) continue;
+        const int kx2 = (N - kx) & (N - 1);
+        const float2 x1 = LX[pad(kx)], x2 = LX[pad(kx2)];
+        const float2 y1 = LY[pad(kx)], y2 = LY[pad(kx2)];
+        float2 AX = make_float2(0.5f * (x1.x + x2.x), 0.5f * (x1.y - x2.y));
+        float2 BX = make_float2(0.5f * (x1.y + x2.y), 0.5f * (x2.x - x1.x));
+        float2 AY = make_float2(0.5f * (y1.x + y2.x), 0.5f * (y1.y - y2.y));
+        float2 BY = make_float2(0.5f * (y1.y + y2.y), 0.5f * (y2.x - y1.x));
+        apply_mode<N, MODE>(AX, AY, kx, 0, p);
+        apply_mode<N, MODE>(BX, BY, kx, HN, p);
+        LX[pad(kx)] = make_float2((AX.x - BX.y) * scale, (AX.y + BX.x) * scale);
+        LY[pad(kx)] = make_float2((AY.x - BY.y) * scale, (AY.y + BY.x) * scale);
+        if (kx2 != kx) {
+          LX[pad(kx2)] = make_float2((AX.x + BX.y) * scale, (BX.x - AX.y) * scale);
+          LY[pad(kx2)] = make_float2((AY.x + BY.y) * scale, (BY.x - AY.y) * scale);
+        }
+      } else {
+        float2 X = LX[pad(kx)], Y = LY[pad(kx)];
+        apply_mode<N, MODE>(X, Y, kx, ky, p);
+        LX[pad(kx)] = make_float2(X.x * scale, X.y * scale);
+        LY[pad(kx)] = make_float2(Y.x * scale, Y.y * scale);
+      }
+    }
+    __syncthreads();
+    fft_lines<N, +1, NL, T, NL>(lds, 0, tid);
+    for (int e = tid; e < N * W; e += T) {
+      const int x = e / W, kw = e % W;
+      const float2 X = lds[(2 * kw) * NP + pad(x)], Y = lds[(2 * kw + 1) * NP + pad(x)];
+      H[(size_t)x * HN + kb * W + kw] = make_float4(X.x, X.y, Y.x, Y.y);
+    }
+    __syncthreads();
   }
 }
 
@@ -523,7 +695,7 @@ namespace {
 template <int KIND, int SCHEME>
 hipError_t go_row(const RowArgs& a, int batch, hipStream_t s) {
   const int bands = NN_ / (RowCfg<NN_>::G * RowCfg<NN_>::ITER);
-  hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(ROW_T), 0, s, a);
+  hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(RowCfg<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 
@@ -550,8 +722,7 @@ hipError_t NSK_CAT(launch_row_, NSK_N)(int kind, int scheme, const RowArgs& a, i
 namespace {
 template <int MODE>
 hipError_t go_col(const ColArgs& a, int batch, hipStream_t s) {
-  const int blocks = (NN_ / 2) / ColCfg<NN_>::W;
-  hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(blocks, batch), dim3(ColCfg<NN_>::T), 0, s, a);
+  hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColCfg<NN_>::NWG, batch), dim3(ColCfg<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 
