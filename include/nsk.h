@@ -69,9 +69,9 @@ typedef struct nsk_slot {
   float dt;
   float A;
   float B;
-  int step;   /* base step index; kernels use step + per-launch offset */
-  int nsteps; /* total steps of this slot's rollout */
-  int pad0;
+  int step;    /* base step index; kernels use step + per-launch offset */
+  int nsteps;  /* total steps of this slot's rollout */
+  float jseed; /* adjoint only: dL/dJ of this slot (multiplies the call's jseed) */
   int pad1;
 } nsk_slot;
 
@@ -166,6 +166,19 @@ int nsk_rollout_grad(const nsk_plan *p, const float *u0, float *u_final,
                      void *spec0, void *spec1, void *spec2, void *spec3,
                      double *jpart, double *gpart, float *ckpt, float *seg,
                      int ckpt_every, void *stream);
+
+/* The adjoint sweep alone, from the checkpoints a previous nsk_rollout
+ * stored (same slots, steps, scheme, objective, ckpt_every; slots[b].step is
+ * `steps` on entry and on return).  Seeds: jseed = dL/dJ (the objective's
+ * terms are scaled by it) and lam_final (nullable) = dL/du_final as (2,N,N)
+ * per slot.  Outputs dtheta_out (nullable, 2 doubles per slot) and du0_out
+ * (nullable).  spec0..spec3, gpart and seg as for nsk_rollout_grad. */
+int nsk_rollout_adjoint(const nsk_plan *p, nsk_slot *slots, int steps, int scheme,
+                        int objective, float jseed, const float *lam_final,
+                        double *dtheta_out, float *du0_out, void *spec0,
+                        void *spec1, void *spec2, void *spec3, double *gpart,
+                        const float *ckpt, float *seg, int ckpt_every,
+                        void *stream);
 
 /* ---- continuous batching building blocks ------------------------------ */
 /* Enters p->batch consecutive slots (pointers already offset to the first

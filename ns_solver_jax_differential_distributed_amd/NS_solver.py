@@ -520,20 +520,24 @@ class SingleCoreNSSolver:
         self.objective_kind = objective
         self.ckpt_every = int(ckpt_every)
         self.use_graph = bool(use_graph)
-        self._ws = {}
+        self._engine = None
         self.jit_objective = self.objective
         self.jit_gradient = self.gradient
 
-    # -- workspace (kept across calls so graph replays see stable pointers)
-    def _buf(self, name, shape, dtype=torch.float32):
-        t = self._ws.get(name)
-        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype or t.device != _device():
-            t = torch.empty(shape, dtype=dtype, device=_device())
-            self._ws[name] = t
-        return t
+    @property
+    def engine(self):
+        from .engine import RolloutEngine
+        if self._engine is None:
+            self._engine = RolloutEngine(self.grid.N, self.grid.L, 1, self.phys.mu0, self.phys.sigma0,
+                                         self.phys.C, self.scheme, self.objective_kind, self.use_graph)
+        return self._engine
+
+    @property
+    def _ws(self):
+        return self.engine._ws
 
     def _plan(self) -> _lib.Plan:
-        return _plan(self.grid.N, self.grid.L, self.phys.mu0, self.phys.sigma0, self.phys.C)
+        return self.engine.plan
 
     def step(self, u, theta, t):
         """Single time step with the Gaussian forcing at time t (NS_solver.py:727-730)."""
@@ -542,61 +546,36 @@ class SingleCoreNSSolver:
     def rollout(self, u0, theta, steps: Optional[int] = None):
         """Integrate self.time.steps steps from u0 accumulating
         J = sum_i dt * sum(u_i . f_i) * dx^2 before each step
-        (NS_solver.py:732-750).  Returns (J, u_final).  ``steps`` overrides
-        the step count for this call (extension)."""
-        N = self.grid.N
+        (NS_solver.py:732-750).  Returns (J, u_final); J is a 0-d float64
+        tensor.  ``steps`` overrides the step count for this call."""
         steps = int(self.time.steps if steps is None else steps)
-        u0 = _field(u0, (2, N, N))
-        A, B = _theta_ab(theta)
-        p = self._plan()
-        slots = _lib.make_slots(self.phys.nu, self.time.dt, A, B, 0, steps)
-        s0 = self._buf("spec0", (1, N, N // 2, 4))
-        s1 = self._buf("spec1", (1, N, N // 2, 4))
-        nb = max(1, int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), max(steps, 1))))
-        jpart = self._buf("jpart", (nb,), torch.float64)
-        J = torch.empty((1,), dtype=torch.float64, device=u0.device)
-        u_final = torch.empty_like(u0)
-        _lib.check(_lib.lib().nsk_rollout(
-            ctypes.byref(p), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
-            _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective_kind], J.data_ptr(),
-            s0.data_ptr(), s1.data_ptr(), jpart.data_ptr(), None, 1, int(self.use_graph),
-            _lib.stream_ptr()), "rollout")
-        return J[0], u_final
+        _theta_ab(theta)
+        J, uf, _ = self.engine.rollout(u0, self.phys.nu, self.time.dt, theta, steps)
+        return J[0], uf[0]
 
     def rollout_and_grad(self, u0, theta):
         """(J, u_final, dJ/dtheta, dJ/du0) by the hand-written discrete
-        adjoint with checkpoints every ``ckpt_every`` steps."""
-        N, steps = self.grid.N, int(self.time.steps)
-        u0 = _field(u0, (2, N, N))
+        adjoint with checkpoints every ``ckpt_every`` steps; dJ/dtheta has
+        the length of theta (entries past the second are exactly 0)."""
         th = np.asarray(theta.detach().cpu() if isinstance(theta, torch.Tensor) else theta,
-                        dtype=np.float64).reshape(-1)
-        A, B = _theta_ab(th)
-        if steps < 1:
-            z = torch.zeros((), dtype=torch.float64, device=u0.device)
-            return z, u0.clone(), torch.zeros(len(th), dtype=torch.float64, device=u0.device), \
-                torch.zeros_like(u0)
-        k = min(self.ckpt_every, steps)
-        ncheck = (steps + k - 1) // k
-        p = self._plan()
-        slots = _lib.make_slots(self.phys.nu, self.time.dt, A, B, 0, steps)
-        specs = [self._buf(f"spec{i}", (1, N, N // 2, 4)) for i in range(4)]
-        nb = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), steps))
-        jpart = self._buf("jpart", (nb,), torch.float64)
-        gpart = self._buf("gpart", (2 * nb,), torch.float64)
-        ckpt = self._buf("ckpt", (ncheck, 2, N, N))
-        seg = self._buf("seg", (k, 2, N, N))
-        J = torch.empty((1,), dtype=torch.float64, device=u0.device)
-        dth = torch.empty((2,), dtype=torch.float64, device=u0.device)
-        du0 = torch.empty_like(u0)
-        u_final = torch.empty_like(u0)
-        _lib.check(_lib.lib().nsk_rollout_grad(
-            ctypes.byref(p), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
-            _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective_kind], J.data_ptr(),
-            dth.data_ptr(), du0.data_ptr(), *[s.data_ptr() for s in specs], jpart.data_ptr(),
-            gpart.data_ptr(), ckpt.data_ptr(), seg.data_ptr(), k, _lib.stream_ptr()), "rollout_grad")
-        g = torch.zeros(len(th), dtype=torch.float64, device=u0.device)
-        g[:2] = dth
-        return J[0], u_final, g, du0
+                        dtype=np.float32).reshape(-1)
+        _theta_ab(th)
+        J, uf, dth, du0 = self.engine.rollout_and_grad(u0, self.phys.nu, self.time.dt, th,
+                                                       int(self.time.steps), self.ckpt_every)
+        g = torch.zeros(len(th), dtype=torch.float64, device=J.device)
+        g[:2] = dth[0]
+        return J[0], uf[0], g, du0[0]
+
+    def rollout_autograd(self, u0: torch.Tensor, theta: torch.Tensor):
+        """(J, u_final) as a differentiable torch function of u0 and theta
+        (backward = the discrete adjoint; see engine.RolloutFunction)."""
+        from .engine import RolloutFunction
+        if not isinstance(u0, torch.Tensor):
+            u0 = _field(u0, (2, self.grid.N, self.grid.N))
+        elif u0.device != _device() or u0.dtype != torch.float32:
+            u0 = u0.to(device=_device(), dtype=torch.float32)   # differentiable move
+        return RolloutFunction.apply(u0, theta, self.engine, self.phys.nu, self.time.dt,
+                                     int(self.time.steps), self.ckpt_every)
 
     def objective(self, theta):
         """J(theta) from the zero initial condition (NS_solver.py:752-756)."""
@@ -611,3 +590,34 @@ class SingleCoreNSSolver:
         u0 = torch.zeros((2, self.grid.N, self.grid.N), dtype=torch.float32, device=_device())
         _, _, g, _ = self.rollout_and_grad(u0, theta)
         return g
+
+
+class BatchedNSSolver:
+    """B independent trajectories on one N x N grid, advanced together by the
+    same kernels (one launch per pass for all slots).  Each slot has its own
+    u0, theta, nu and dt; the step count is shared.  Not in the reference:
+    it batches what SingleCoreNSSolver does per trajectory, with results
+    identical to B separate SingleCoreNSSolver runs."""
+
+    def __init__(self, N: int, L: float, batch: int, phys: Optional[Phys] = None,
+                 advection_scheme: str = "central", objective: str = "work", ckpt_every: int = 50,
+                 use_graph: bool = True):
+        from .engine import RolloutEngine
+        phys = phys or Phys(nu=0.01, mu0=0.5, sigma0=0.1, C=1.0)
+        self.N, self.L, self.batch, self.phys = N, L, batch, phys
+        self.ckpt_every = int(ckpt_every)
+        self.engine = RolloutEngine(N, L, batch, phys.mu0, phys.sigma0, phys.C, advection_scheme,
+                                    objective, use_graph)
+
+    def rollout(self, u0, theta, nu, dt, steps: int):
+        """(J (B,) float64, u_final (B,2,N,N)) for per-slot nu, dt (scalars or (B,))."""
+        J, uf, _ = self.engine.rollout(u0, nu, dt, theta, steps)
+        return J, uf
+
+    def rollout_and_grad(self, u0, theta, nu, dt, steps: int):
+        """(J, u_final, dJ/dtheta (B,2), dJ/du0 (B,2,N,N))."""
+        return self.engine.rollout_and_grad(u0, nu, dt, theta, steps, self.ckpt_every)
+
+    def rollout_autograd(self, u0: torch.Tensor, theta: torch.Tensor, nu, dt, steps: int):
+        from .engine import RolloutFunction
+        return RolloutFunction.apply(u0, theta, self.engine, nu, dt, int(steps), self.ckpt_every)

@@ -6,14 +6,14 @@ The solver API lives in ``ns_solver_jax_differential_distributed_amd.NS_solver``
 and is re-exported here.
 """
 from .NS_solver import (  # noqa: F401
-    CONFIG, AdvectionScheme, CentralAdvection, ChorinProjector, DifferentialOps,
+    CONFIG, BatchedNSSolver, AdvectionScheme, CentralAdvection, ChorinProjector, DifferentialOps,
     GaussianForcingField, Grid, PeriodicDifferentialOps, Phys, SimConfig,
     SingleCoreNSSolver, SpectralPoissonSolver, Time, UpwindAdvection, load_config,
     make_advection_scheme, make_grid, setup_logger,
 )
 
 __all__ = [
-    "CONFIG", "AdvectionScheme", "CentralAdvection", "ChorinProjector", "DifferentialOps",
+    "CONFIG", "BatchedNSSolver", "AdvectionScheme", "CentralAdvection", "ChorinProjector", "DifferentialOps",
     "GaussianForcingField", "Grid", "PeriodicDifferentialOps", "Phys", "SimConfig",
     "SingleCoreNSSolver", "SpectralPoissonSolver", "Time", "UpwindAdvection", "load_config",
     "make_advection_scheme", "make_grid", "setup_logger",

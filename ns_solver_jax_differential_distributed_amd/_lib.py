@@ -31,7 +31,7 @@ EXPORTED = (
     "nsk_partials_per_slot", "nsk_grad", "nsk_laplacian", "nsk_divergence",
     "nsk_advect", "nsk_forcing", "nsk_poisson", "nsk_project", "nsk_step",
     "nsk_rollout", "nsk_rollout_grad", "nsk_admit", "nsk_advance",
-    "nsk_finalize", "nsk_column_pass", "nsk_row_pass",
+    "nsk_finalize", "nsk_column_pass", "nsk_row_pass", "nsk_rollout_adjoint",
 )
 
 
@@ -47,7 +47,7 @@ class Plan(ctypes.Structure):
     ]
 
 
-SLOT_WORDS = 8  # nsk_slot: nu, dt, A, B (float) | step, nsteps, pad, pad (int)
+SLOT_WORDS = 8  # nsk_slot: nu, dt, A, B (float) | step, nsteps (int) | jseed (float) | pad
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -77,6 +77,8 @@ _SIGS = {
     "nsk_finalize": ([_P, _vp, _vp, _i, _i, _vp, _vp], _i),
     "nsk_column_pass": ([_P, _vp, _i, _vp], _i),
     "nsk_row_pass": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp], _i),
+    "nsk_rollout_adjoint": ([_P, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _i, _vp], _i),
 }
 
 _lib = None
@@ -139,17 +141,24 @@ def ptr(t) -> int | None:
     return t.data_ptr()
 
 
-def make_slots(nu, dt, A, B, step=0, nsteps=0, batch: int = 1, device=None) -> torch.Tensor:
-    """Device table of ``nsk_slot`` records, one per slot (int32 words; the
-    float fields are reinterpreted in place)."""
+def slot_records(nu, dt, A, B, step=0, nsteps=0, batch: int = 1, jseed=1.0):
+    """Host (numpy int32) table of ``nsk_slot`` records, one row per slot;
+    the float fields are reinterpreted in place."""
     import numpy as np
     rec = np.zeros((batch, SLOT_WORDS), dtype=np.int32)
     f = rec.view(np.float32)
+    f[:, 6] = np.broadcast_to(np.asarray(jseed, dtype=np.float32), (batch,))
     f[:, 0] = np.broadcast_to(np.asarray(nu, dtype=np.float32), (batch,))
     f[:, 1] = np.broadcast_to(np.asarray(dt, dtype=np.float32), (batch,))
     f[:, 2] = np.broadcast_to(np.asarray(A, dtype=np.float32), (batch,))
     f[:, 3] = np.broadcast_to(np.asarray(B, dtype=np.float32), (batch,))
     rec[:, 4] = np.broadcast_to(np.asarray(step, dtype=np.int32), (batch,))
     rec[:, 5] = np.broadcast_to(np.asarray(nsteps, dtype=np.int32), (batch,))
+    return rec
+
+
+def make_slots(nu, dt, A, B, step=0, nsteps=0, batch: int = 1, device=None, jseed=1.0) -> torch.Tensor:
+    """Device table of ``nsk_slot`` records (see slot_records)."""
+    rec = slot_records(nu, dt, A, B, step, nsteps, batch, jseed)
     dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
     return torch.from_numpy(rec).to(dev)

@@ -114,6 +114,7 @@ RowArgs row_args(const nsk_plan* p, const nsk_slot* slots) {
   a.dst_nc = 2;
   a.t_explicit = -1.0f;
   a.st_every = 1;
+  a.jseed = 1.0f;
   return a;
 }
 
@@ -643,26 +644,22 @@ int nsk_rollout(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* sl
                       ckpt_every, use_graph != 0, (hipStream_t)stream);
 }
 
-int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
-                     int scheme, int objective, double* J_out, double* dtheta_out, float* du0_out,
-                     void* spec0, void* spec1, void* spec2, void* spec3, double* jpart, double* gpart,
-                     float* ckpt, float* seg, int ckpt_every, void* stream) {
-  CHECK_PLAN(p);
-  int rc = check_common(p, steps, scheme, objective);
-  if (rc) return rc;
-  if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_grad needs steps >= 1");
-  if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
-  if (!u0 || !u_final || !slots || !spec0 || !spec1 || !spec2 || !spec3 || !jpart || !gpart || !ckpt ||
-      !seg || !dtheta_out)
-    return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
-  hipStream_t st = (hipStream_t)stream;
+static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int scheme, int objective,
+                        float jseed, const float* lam_final, double* dtheta_out, float* du0_out,
+                        void* spec0, void* spec1, void* spec2, void* spec3, double* gpart,
+                        const float* ckpt, float* seg, int ckpt_every, hipStream_t st) {
   const long long NN2 = 2LL * p->N * p->N;
   const int ncheck = (steps + ckpt_every - 1) / ckpt_every;
-  rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
-                    ckpt_every, false, st);
-  if (rc) return rc;
-  // adjoint: lambda_steps = 0
-  HIP_TRY(zero_spec(p, spec2, st));
+  // lambda_steps: the terminal seed (row spectrum of dL/du_final) or 0
+  if (lam_final) {
+    RowArgs r = row_args(p, nullptr);
+    r.src_real = lam_final;
+    r.src_bstride = NN2;
+    r.dst_spec = (float4*)spec2;
+    HIP_TRY(launch_row(p->N, RK_REAL_TO_SPEC, 0, r, p->batch, st));
+  } else {
+    HIP_TRY(zero_spec(p, spec2, st));
+  }
   void* lam = spec2;
   void* lam_next = spec3;
   for (int sg = ncheck - 1; sg >= 0; --sg) {
@@ -688,10 +685,8 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
       RowArgs q = row_args(p, slots);
       q.src_spec = (const float4*)cur;
       q.dst_spec = (float4*)nxt;
-      q.dst_real = u_final;  // never written: n < s1 <= steps
-      q.dst_bstride = NN2;
       q.step_off = m + 1 - s0;
-      q.require = REQ_ROLL;
+      q.require = REQ_ACTIVE;
       q.ustore = seg;
       q.st_bstride = (long long)ckpt_every * NN2;
       q.st_base = s0;
@@ -712,6 +707,7 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
       a.part = gpart;
       a.pcap = steps;
       a.objective = objective;
+      a.jseed = jseed;
       a.ustate = seg;
       a.us_bstride = (long long)ckpt_every * NN2;
       a.us_base = s0;
@@ -731,8 +727,43 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
     w.dst_bstride = NN2;
     HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
   }
-  HIP_TRY(finalize(p, slots, gpart, steps, 2, dtheta_out, st));
+  if (dtheta_out) HIP_TRY(finalize(p, slots, gpart, steps, 2, dtheta_out, st));
   return NSK_OK;
+}
+
+int nsk_rollout_adjoint(const nsk_plan* p, nsk_slot* slots, int steps, int scheme, int objective,
+                        float jseed, const float* lam_final, double* dtheta_out, float* du0_out,
+                        void* spec0, void* spec1, void* spec2, void* spec3, double* gpart,
+                        const float* ckpt, float* seg, int ckpt_every, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, steps, scheme, objective);
+  if (rc) return rc;
+  if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_adjoint needs steps >= 1");
+  if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
+  if (!slots || !spec0 || !spec1 || !spec2 || !spec3 || !gpart || !ckpt || !seg)
+    return fail(NSK_E_INVAL, "nsk_rollout_adjoint: NULL buffer");
+  return adjoint_impl(p, slots, steps, scheme, objective, jseed, lam_final, dtheta_out, du0_out, spec0, spec1,
+                      spec2, spec3, gpart, ckpt, seg, ckpt_every, (hipStream_t)stream);
+}
+
+int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
+                     int scheme, int objective, double* J_out, double* dtheta_out, float* du0_out,
+                     void* spec0, void* spec1, void* spec2, void* spec3, double* jpart, double* gpart,
+                     float* ckpt, float* seg, int ckpt_every, void* stream) {
+  CHECK_PLAN(p);
+  int rc = check_common(p, steps, scheme, objective);
+  if (rc) return rc;
+  if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_grad needs steps >= 1");
+  if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
+  if (!u0 || !u_final || !slots || !spec0 || !spec1 || !spec2 || !spec3 || !jpart || !gpart || !ckpt ||
+      !seg || !dtheta_out)
+    return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
+  hipStream_t st = (hipStream_t)stream;
+  rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
+                    ckpt_every, false, st);
+  if (rc) return rc;
+  return adjoint_impl(p, slots, steps, scheme, objective, 1.0f, nullptr, dtheta_out, du0_out, spec0, spec1,
+                      spec2, spec3, gpart, ckpt, seg, ckpt_every, st);
 }
 
 int nsk_admit(const nsk_plan* p, const float* u0, nsk_slot* slots, int scheme, int objective, void* spec,

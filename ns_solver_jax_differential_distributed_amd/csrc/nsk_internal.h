@@ -42,10 +42,12 @@ struct RowArgs {
   float* ustore;
   long long st_bstride;
   int st_base, st_every, st_cap;
-  // adjoint: forward state u_n at ustate[n - us_base]
+  // adjoint: forward state u_n at ustate[n - us_base]; jseed scales the
+  // objective's direct terms (the incoming dL/dJ)
   const float* ustate;
   long long us_bstride;
   int us_base;
+  float jseed;
 };
 
 struct ColArgs {

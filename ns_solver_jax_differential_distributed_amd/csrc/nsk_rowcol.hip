@@ -298,7 +298,8 @@ __global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
     sl = a.slots[b];
   } else {
     sl.nu = sl.dt = sl.A = sl.B = 0.0f;
-    sl.step = sl.nsteps = sl.pad0 = sl.pad1 = 0;
+    sl.step = sl.nsteps = sl.pad1 = 0;
+    sl.jseed = 1.0f;
   }
   const int n = sl.step + a.step_off;
   bool final_mode = false;
@@ -432,9 +433,10 @@ __global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
       // until the phase's reads are done, then written into the slots of
       // rows xb-1+g, which later rows no longer read.
       const float* F = FFIELD ? a.force + (long long)b * a.force_bstride : nullptr;
+      const double seed = (double)a.jseed * (double)sl.jseed;
       const float scale_ke = (a.objective == NSK_OBJ_MEAN_KE)
-                                 ? (float)(1.0 / ((double)sl.nsteps * (double)NN)) : 0.0f;
-      const float jdir = (float)((double)dt * dx2);
+                                 ? (float)(seed / ((double)sl.nsteps * (double)NN)) : 0.0f;
+      const float jdir = (float)(seed * (double)dt * dx2);
 #pragma unroll
       for (int ph = 0; ph < PH; ++ph) {
         float2 res[RES];
@@ -479,7 +481,7 @@ __global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
               double grow;
               if (a.objective == NSK_OBJ_WORK) {
                 v.x += jdir * s_f[g];
-                grow = (double)dt * dx2 * (double)Uc.x + (double)dt * (double)c.x;
+                grow = seed * (double)dt * dx2 * (double)Uc.x + (double)dt * (double)c.x;
               } else {
                 v.x += scale_ke * Uc.x;
                 v.y += scale_ke * Uc.y;

@@ -1,0 +1,215 @@
+"""Device-side rollout engine: one grid size, B independent slots.
+
+Owns the workspace the C ABI needs (row-spectral buffers, objective and
+gradient partial sums, checkpoints, recomputation segment) as torch tensors
+that persist across calls, so HIP-graph replays see stable pointers.  Used by
+``SingleCoreNSSolver`` (B = 1) and ``BatchedNSSolver`` (B trajectories with
+their own nu, dt, theta).
+
+The objective J and the parameter gradient come back in FP64; fields are
+float32 (2, N, N) per slot.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the MI355X kernels need a GPU; torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def check_n(N: int) -> None:
+    if N < 8 or N > 4096 or (N & (N - 1)) != 0:
+        raise ValueError(f"N={N}: the MI355X kernels support powers of two from 8 to 4096")
+
+
+def theta_table(theta, batch: int) -> np.ndarray:
+    """(batch, >=2) float32 host array of parameters."""
+    th = theta.detach().cpu().numpy() if isinstance(theta, torch.Tensor) else np.asarray(theta)
+    th = np.asarray(th, dtype=np.float32)
+    if th.ndim == 1:
+        th = np.broadcast_to(th, (batch, th.shape[0]))
+    if th.shape[0] != batch or th.shape[1] < 2:
+        raise ValueError(f"theta must be (>=2,) or ({batch}, >=2), got {th.shape}")
+    return np.ascontiguousarray(th)
+
+
+def per_slot(v, batch: int, name: str) -> np.ndarray:
+    a = np.asarray(v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else v, dtype=np.float64)
+    if a.ndim == 0:
+        a = np.full(batch, float(a))
+    if a.shape != (batch,):
+        raise ValueError(f"{name} must be a scalar or ({batch},), got {a.shape}")
+    return a
+
+
+class RolloutEngine:
+    """Plan + workspace for B slots of an N x N periodic grid of length L."""
+
+    def __init__(self, N: int, L: float, batch: int = 1, mu0: float = 0.5, sigma0: float = 0.1,
+                 C: float = 1.0, scheme: str = "central", objective: str = "work",
+                 use_graph: bool = True):
+        check_n(N)
+        if scheme not in _lib.SCHEME:
+            raise ValueError(f"Unknown advection scheme: '{scheme}'. Choose 'central' or 'upwind'.")
+        if objective not in _lib.OBJECTIVE:
+            raise ValueError(f"objective must be 'work' or 'mean_ke', got {objective!r}")
+        if batch < 1:
+            raise ValueError("batch must be >= 1")
+        self.N, self.L, self.batch = int(N), float(L), int(batch)
+        self.phys = (float(mu0), float(sigma0), float(C))
+        self.scheme, self.objective = scheme, objective
+        self.use_graph = bool(use_graph)
+        self.plan = _lib.make_plan(N, L, mu0, sigma0, C, batch)
+        self._ws = {}
+
+    # ---------------------------------------------------------------- buffers
+    def buf(self, name, shape, dtype=torch.float32) -> torch.Tensor:
+        t = self._ws.get(name)
+        dev = _device()
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype or t.device != dev:
+            t = torch.empty(shape, dtype=dtype, device=dev)
+            self._ws[name] = t
+        return t
+
+    def spec(self, i: int) -> torch.Tensor:
+        return self.buf(f"spec{i}", (self.batch, self.N, self.N // 2, 4))
+
+    def bands(self) -> int:
+        return int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(self.plan), 1))
+
+    def slots(self, nu, dt, theta, steps: int, jseed=1.0) -> torch.Tensor:
+        th = theta_table(theta, self.batch)
+        rec = _lib.slot_records(per_slot(nu, self.batch, "nu"), per_slot(dt, self.batch, "dt"),
+                                th[:, 0], th[:, 1], 0, steps, self.batch, jseed)
+        t = self.buf("slots", (self.batch, _lib.SLOT_WORDS), torch.int32)
+        t.copy_(torch.from_numpy(rec))
+        return t
+
+    def field(self, u, name="u0") -> torch.Tensor:
+        u = u if isinstance(u, torch.Tensor) else torch.as_tensor(np.asarray(u))
+        u = u.to(device=_device(), dtype=torch.float32)
+        if u.dim() == 3:
+            u = u.unsqueeze(0)
+        if tuple(u.shape) != (self.batch, 2, self.N, self.N):
+            raise ValueError(f"{name} must be (2,{self.N},{self.N}) or ({self.batch},2,{self.N},{self.N}), "
+                             f"got {tuple(u.shape)}")
+        return u.contiguous()
+
+    # ---------------------------------------------------------------- forward
+    def rollout(self, u0, nu, dt, theta, steps: int, ckpt_every: Optional[int] = None):
+        """(J (B,) float64, u_final (B,2,N,N), ckpt or None).  With
+        ckpt_every, u_n is stored every ckpt_every steps for the adjoint."""
+        steps = int(steps)
+        if steps < 0:
+            raise ValueError("steps must be >= 0")
+        u0 = self.field(u0)
+        slots = self.slots(nu, dt, theta, steps)
+        nb = self.bands()
+        jpart = self.buf("jpart", (self.batch * max(steps, 1) * nb,), torch.float64)
+        J = torch.empty((self.batch,), dtype=torch.float64, device=u0.device)
+        u_final = torch.empty_like(u0)
+        ckpt = None
+        k = 1
+        if ckpt_every is not None and steps > 0:
+            k = max(1, min(int(ckpt_every), steps))
+            ncheck = (steps + k - 1) // k
+            ckpt = torch.empty((self.batch, ncheck, 2, self.N, self.N), dtype=torch.float32, device=u0.device)
+        _lib.check(_lib.lib().nsk_rollout(
+            ctypes.byref(self.plan), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
+            _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective], J.data_ptr(),
+            self.spec(0).data_ptr(), self.spec(1).data_ptr(), jpart.data_ptr(),
+            None if ckpt is None else ckpt.data_ptr(), k, int(self.use_graph and ckpt is None),
+            _lib.stream_ptr()), "rollout")
+        return J, u_final, ckpt
+
+    # ---------------------------------------------------------------- adjoint
+    def adjoint(self, nu, dt, theta, steps: int, ckpt: torch.Tensor, ckpt_every: int,
+                jseed=1.0, lam_final: Optional[torch.Tensor] = None, want_du0: bool = True):
+        """Reverse sweep from checkpoints of a previous rollout.  Returns
+        (dJ/dtheta (B,2) float64, dJ/du0 (B,2,N,N) or None) for the seeds
+        jseed = dL/dJ (scalar or (B,)) and lam_final = dL/du_final."""
+        steps = int(steps)
+        k = max(1, min(int(ckpt_every), steps))
+        slots = self.slots(nu, dt, theta, steps, per_slot(jseed, self.batch, "jseed"))
+        # the adjoint expects slots[b].step == steps on entry
+        slots[:, 4] = steps
+        nb = self.bands()
+        gpart = self.buf("gpart", (self.batch * steps * nb * 2,), torch.float64)
+        seg = self.buf("seg", (self.batch, k, 2, self.N, self.N))
+        dth = torch.empty((self.batch, 2), dtype=torch.float64, device=ckpt.device)
+        du0 = torch.empty((self.batch, 2, self.N, self.N), dtype=torch.float32,
+                          device=ckpt.device) if want_du0 else None
+        lf = None if lam_final is None else self.field(lam_final, "lam_final")
+        _lib.check(_lib.lib().nsk_rollout_adjoint(
+            ctypes.byref(self.plan), slots.data_ptr(), steps, _lib.SCHEME[self.scheme],
+            _lib.OBJECTIVE[self.objective], 1.0, None if lf is None else lf.data_ptr(),
+            dth.data_ptr(), None if du0 is None else du0.data_ptr(),
+            *[self.spec(i).data_ptr() for i in range(4)], gpart.data_ptr(), ckpt.data_ptr(),
+            seg.data_ptr(), k, _lib.stream_ptr()), "rollout_adjoint")
+        return dth, du0
+
+    def rollout_and_grad(self, u0, nu, dt, theta, steps: int, ckpt_every: int = 50):
+        """(J, u_final, dJ/dtheta (B,2), dJ/du0)."""
+        steps = int(steps)
+        if steps < 1:
+            u0 = self.field(u0)
+            z = torch.zeros((self.batch,), dtype=torch.float64, device=u0.device)
+            return z, u0.clone(), torch.zeros((self.batch, 2), dtype=torch.float64,
+                                              device=u0.device), torch.zeros_like(u0)
+        J, uf, ckpt = self.rollout(u0, nu, dt, theta, steps, ckpt_every)
+        dth, du0 = self.adjoint(nu, dt, theta, steps, ckpt, ckpt_every)
+        return J, uf, dth, du0
+
+
+class RolloutFunction(torch.autograd.Function):
+    """J, u_final = rollout(u0, theta) with a hand-written discrete adjoint.
+
+    forward stores u_n every ``ckpt_every`` steps; backward recomputes each
+    segment from its checkpoint and sweeps the adjoint back through it, with
+    the seeds dL/dJ and dL/du_final.  theta is (>=2,) or (B, >=2); entries past
+    the second get exact zeros (the forcing reads only theta[0], theta[1])."""
+
+    @staticmethod
+    def forward(ctx, u0, theta, engine: RolloutEngine, nu, dt, steps: int, ckpt_every: int):
+        J, uf, ckpt = engine.rollout(u0, nu, dt, theta, steps, ckpt_every if steps > 0 else None)
+        ctx.engine, ctx.nu, ctx.dt, ctx.steps, ctx.k = engine, nu, dt, int(steps), ckpt_every
+        ctx.theta_shape, ctx.theta_dtype = tuple(theta.shape), theta.dtype
+        ctx.u0_shape = tuple(u0.shape)
+        ctx.save_for_backward(theta.detach(), ckpt if ckpt is not None else torch.empty(0))
+        if u0.dim() == 3:
+            return J[0], uf[0]
+        return J, uf
+
+    @staticmethod
+    def backward(ctx, gJ, guf):
+        theta, ckpt = ctx.saved_tensors
+        eng = ctx.engine
+        B = eng.batch
+        if ctx.steps < 1:
+            gu0 = guf if guf is not None else None
+            return gu0, torch.zeros(ctx.theta_shape, dtype=ctx.theta_dtype, device=theta.device), \
+                None, None, None, None, None
+        seed = np.ones(B) if gJ is None else gJ.detach().double().reshape(-1).cpu().numpy()
+        lam = None
+        if guf is not None:
+            lam = guf.detach().reshape(B, 2, eng.N, eng.N)
+        dth, du0 = eng.adjoint(ctx.nu, ctx.dt, theta, ctx.steps, ckpt, ctx.k, seed, lam,
+                               want_du0=ctx.needs_input_grad[0])
+        gtheta = None
+        if ctx.needs_input_grad[1]:
+            g = torch.zeros((B, ctx.theta_shape[-1]), dtype=torch.float64, device=theta.device)
+            g[:, :2] = dth
+            gtheta = g.reshape(ctx.theta_shape).to(ctx.theta_dtype)
+        gu0 = None
+        if du0 is not None:
+            gu0 = du0.reshape(ctx.u0_shape)
+        return gu0, gtheta, None, None, None, None, None

@@ -103,8 +103,9 @@ def forcing_theta_partials(theta, t, x, phys: O.PhysParams):
 
 
 def rollout_gradient(u0, theta, N, L, phys: O.PhysParams, dt, steps,
-                     scheme="central", objective="work"):
-    """Returns (J, u_final, dJ/dtheta (len(theta)), dJ/du0) in FP64."""
+                     scheme="central", objective="work", jseed=1.0, lam_final=None):
+    """Returns (J, u_final, dL/dtheta (len(theta)), dL/du0) in FP64 for
+    L = jseed * J + <lam_final, u_final> (jseed = 1, lam_final = 0: dJ)."""
     u0 = np.asarray(u0, dtype=np.float64)
     dx = L / N
     x = O.grid_x(N, L, np.float64)
@@ -112,7 +113,7 @@ def rollout_gradient(u0, theta, N, L, phys: O.PhysParams, dt, steps,
                                    objective, keep_states=True)
     theta = np.asarray(theta, dtype=np.float64)
     g_theta = np.zeros(len(theta))
-    lam = np.zeros_like(u0)
+    lam = np.zeros_like(u0) if lam_final is None else np.asarray(lam_final, dtype=np.float64).copy()
     for n in range(steps - 1, -1, -1):
         t = float(n) * dt
         un = states[n]
@@ -120,10 +121,10 @@ def rollout_gradient(u0, theta, N, L, phys: O.PhysParams, dt, steps,
         mu = projection(lam, dx)
         if objective == "work":
             direct_u = np.zeros_like(un)
-            direct_u[0] = dt * dx * dx * fx[:, None]
-            gf = dt * dx * dx * un[0] + dt * mu[0]
+            direct_u[0] = jseed * dt * dx * dx * fx[:, None]
+            gf = jseed * dt * dx * dx * un[0] + dt * mu[0]
         else:
-            direct_u = un / (steps * N * N)
+            direct_u = jseed * un / (steps * N * N)
             gf = dt * mu[0]
         dA, dB = forcing_theta_partials(theta, t, x, phys)
         row = gf.sum(axis=1)

@@ -1,0 +1,99 @@
+"""Batched trajectories and the torch.autograd path (discrete adjoint behind
+RolloutFunction) on the GPU."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_oracle as AO
+from oracle import ns_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def S():
+    from ns_solver_jax_differential_distributed_amd import NS_solver
+    return NS_solver
+
+
+def W():
+    from ns_solver_jax_differential_distributed_amd import workloads
+    return workloads
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("N,B,scheme", [(64, 5, "central"), (256, 3, "upwind")])
+def test_batched_slots_equal_separate_runs_bitwise(N, B, scheme):
+    """Each slot of a batched rollout (own u0, theta, nu, dt) is bit-identical
+    to the same trajectory run alone."""
+    L = 2 * math.pi
+    u0 = np.stack([W().initial_velocity(N, seed=s, kmax=4) for s in range(B)])
+    th = W().theta_batch(B, seed=1)
+    nu = W().nu_batch(B, seed=1)
+    dt = np.array([W().cfl_dt(u0[b], L, 0.3, float(nu[b])) for b in range(B)], np.float32)
+    steps = 37
+    bs = S().BatchedNSSolver(N, L, B, advection_scheme=scheme)
+    J, uf = bs.rollout(torch.from_numpy(u0), th, nu, dt, steps)
+    for b in range(B):
+        one = S().SingleCoreNSSolver(S().make_grid(N, L), S().Phys(float(nu[b]), 0.5, 0.1, 1.0),
+                                     S().Time(float(dt[b]), steps), advection_scheme=scheme)
+        Jb, ub = one.rollout(torch.from_numpy(u0[b]), th[b])
+        assert torch.equal(ub, uf[b]) and float(Jb) == float(J[b])
+
+
+def test_batched_gradient_matches_oracle():
+    N, B, L, steps = 32, 3, 1.0, 17
+    rng = np.random.default_rng(3)
+    u0 = (0.3 * rng.standard_normal((B, 2, N, N))).astype(np.float32)
+    th = W().theta_batch(B, seed=4)
+    nu = np.array([0.01, 0.005, 0.02], np.float32)
+    dt = np.array([1e-3, 2e-3, 1.5e-3], np.float32)
+    bs = S().BatchedNSSolver(N, L, B, ckpt_every=4)
+    J, uf, g, du0 = bs.rollout_and_grad(torch.from_numpy(u0), th, nu, dt, steps)
+    for b in range(B):
+        Jo, uo, go, lo = AO.rollout_gradient(u0[b].astype(np.float64), th[b], N, L,
+                                             O.PhysParams(float(nu[b])), float(dt[b]), steps)
+        assert abs(float(J[b]) - Jo) / abs(Jo) < 1e-5
+        assert np.max(np.abs(g[b].cpu().numpy() - go[:2])) / np.max(np.abs(go[:2])) < 1e-4
+        assert rel(du0[b].cpu().numpy(), lo) < 1e-4
+
+
+@pytest.mark.parametrize("objective", ["work", "mean_ke"])
+def test_autograd_with_terminal_and_objective_seeds(objective):
+    """L = 2.5*J + <w, u_final> through torch.autograd; gradients w.r.t. u0
+    and theta match the FP64 adjoint oracle with the same seeds."""
+    N, L, nu, dt, steps = 32, 2 * math.pi, 1e-3, 2e-3, 21
+    u0 = W().initial_velocity(N, seed=5, kmax=4)
+    w = np.random.default_rng(6).standard_normal((2, N, N)).astype(np.float32)
+    solver = S().SingleCoreNSSolver(S().make_grid(N, L), S().Phys(nu, 0.5, 0.1, 1.0), S().Time(dt, steps),
+                                    objective=objective, ckpt_every=6)
+    u0_t = torch.tensor(u0, device="cuda", requires_grad=True)
+    th_t = torch.tensor([1.1, 0.7, 0.9], device="cuda", requires_grad=True)
+    J, uf = solver.rollout_autograd(u0_t, th_t)
+    loss = 2.5 * J + (torch.tensor(w, device="cuda") * uf).sum()
+    loss.backward()
+    Jo, uo, go, lo = AO.rollout_gradient(u0.astype(np.float64), [1.1, 0.7, 0.9], N, L, O.PhysParams(nu), dt,
+                                         steps, "central", objective, jseed=2.5, lam_final=w.astype(np.float64))
+    gt = th_t.grad.cpu().numpy()
+    assert gt[2] == 0.0
+    assert np.max(np.abs(gt[:2] - go[:2])) / np.max(np.abs(go[:2])) < 2e-4
+    assert rel(u0_t.grad.cpu().numpy(), lo) < 1e-4
+
+
+def test_autograd_matches_finite_differences_of_the_gpu_rollout():
+    """d J / d theta by the adjoint vs central differences (h = 1e-4, the
+    studies.py:306-318 protocol) of the FP64 oracle over 4 seeded thetas."""
+    N, L, nu, dt, steps = 32, 1.0, 0.01, 1e-3, 30
+    solver = S().SingleCoreNSSolver(S().make_grid(N, L), S().Phys(nu, 0.5, 0.1, 1.0), S().Time(dt, steps),
+                                    ckpt_every=7)
+    for th in W().theta_batch(4, seed=11):
+        g = solver.jit_gradient(th).cpu().numpy()
+        fd = O.fd_gradient(th, 1e-4, lambda t: O.objective(t, N, L, O.PhysParams(nu), dt, steps))
+        assert np.max(np.abs(g[:2] - fd[:2])) / np.max(np.abs(fd[:2])) < 1e-3
+        assert g[2] == 0.0

@@ -267,3 +267,23 @@ def test_projection_is_symmetric():
     rng = np.random.default_rng(7)
     v, w = rng.standard_normal((2, 2, N, N))
     assert abs(np.sum(AO.projection(v, dx) * w) - np.sum(v * AO.projection(w, dx))) < 1e-10
+
+
+def test_adjoint_seeds_match_finite_differences():
+    """L = jseed*J + <w, u_final>: the oracle's seeded adjoint against FD."""
+    N, nu, dt, steps = 10, 0.01, 1e-3, 9
+    rng = np.random.default_rng(8)
+    u0 = 0.4 * rng.standard_normal((2, N, N))
+    w = rng.standard_normal((2, N, N))
+    th = [0.8, 0.6]
+    ph = O.PhysParams(nu)
+
+    def Lf(t, u=u0):
+        J, uf = O.rollout(u, t, N, 1.0, ph, dt, steps)
+        return 1.7 * J + np.sum(w * uf)
+    _, _, g, lam = AO.rollout_gradient(u0, th, N, 1.0, ph, dt, steps, jseed=1.7, lam_final=w)
+    np.testing.assert_allclose(g, O.fd_gradient(th, 1e-5, Lf), rtol=1e-5, atol=1e-10)
+    v = rng.standard_normal(u0.shape)
+    h = 1e-6
+    fdu = (Lf(th, u0 + h * v) - Lf(th, u0 - h * v)) / (2 * h)
+    assert abs(np.sum(lam * v) - fdu) < 1e-6 * max(1.0, abs(fdu))
