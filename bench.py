@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched / gradient / serving legs")
     return ap.parse_args()
 
 
@@ -110,6 +112,125 @@ def pass_times(solver, reps):
     return out
 
 
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def leg_batched(traj=256, N=256, steps=500):
+    """256 independent trajectories at N=256 (own u0, theta, nu, dt), all in
+    one launch per pass, against the same solves run one after another."""
+    import numpy as np
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    u0 = np.stack([W.initial_velocity(N, seed=j, kmax=8) for j in range(traj)])
+    th = W.theta_batch(traj, seed=0)
+    nu = W.nu_batch(traj, seed=0)
+    dt = np.array([W.cfl_dt(u0[j], L, 0.3, float(nu[j]), stable=True) for j in range(traj)], np.float32)
+    bs = S.BatchedNSSolver(N, L, traj)
+    u0d = torch.from_numpy(u0).cuda()
+    bs.rollout(u0d, th, nu, dt, 20)
+    t_b, (J, uf) = _timed(lambda: bs.rollout(u0d, th, nu, dt, steps))
+    plain = [S.SingleCoreNSSolver(S.make_grid(N, L), S.Phys(float(nu[j]), 0.5, 0.1, 1.0),
+                                  S.Time(float(dt[j]), steps)) for j in range(traj)]
+    plain[0].rollout(u0d[0], th[0], steps=20)
+
+    def seq():
+        return [plain[j].rollout(u0d[j], th[j]) for j in range(traj)]
+    t_s, outs = _timed(seq)
+    same = all(torch.equal(outs[j][1], uf[j]) for j in range(traj))
+    return {"workload": f"{traj} trajectories, N={N}, {steps} steps, per-slot nu/theta/dt",
+            "batched_s": t_b, "sequential_s": t_s, "speedup": t_s / t_b,
+            "batched_cell_updates_per_s": traj * N * N * steps / t_b,
+            "solves_per_s": traj / t_b, "bitwise_equal_to_sequential": bool(same)}
+
+
+def leg_gradient(N=512, steps=2000, k=50, nth=16):
+    """Rollout + discrete adjoint, checkpoint every k steps, 16 seeded theta
+    vectors batched; peak HBM of the call."""
+    import numpy as np
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    u0 = W.initial_velocity(N, seed=0, kmax=8)
+    nu = 1e-3
+    dt = W.cfl_dt(u0, L, 0.3, nu, stable=True)
+    th = W.theta_batch(nth, seed=0)
+    bs = S.BatchedNSSolver(N, L, nth, ckpt_every=k)
+    u0d = torch.from_numpy(np.broadcast_to(u0, (nth,) + u0.shape).copy()).cuda()
+    bs.rollout_and_grad(u0d, th, nu, dt, 2 * k)
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    t, (J, uf, g, du0) = _timed(lambda: bs.rollout_and_grad(u0d, th, nu, dt, steps))
+    peak = torch.cuda.max_memory_allocated() - base
+    return {"workload": f"N={N}, {steps} steps, checkpoint every {k}, {nth} theta batched",
+            "seconds": t, "gradients_per_s": nth / t,
+            "cell_steps_per_s_fwd": nth * N * N * steps / t, "peak_hbm_bytes": int(peak),
+            "finite": bool(torch.isfinite(g).all().item())}
+
+
+def leg_serving(n=300, rate=200.0):
+    """Poisson stream (200 req/s) of 70% rollout / 30% rollout+gradient
+    requests, N from {128..2048}, steps in [50, 500]: p50/p99 latency at that
+    rate, and solves/s at saturation against plain per-request execution."""
+    import numpy as np
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import engine as E
+    from ns_solver_jax_differential_distributed_amd import serving as SV
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+
+    def make():
+        reqs = SV.poisson_requests(n, rate, seed=0)
+        for r in reqs:
+            r.u0 = torch.from_numpy(W.initial_velocity(r.N, seed=r.seed, kmax=min(8, r.N // 2 - 1))).cuda()
+        return reqs
+    reqs = make()
+    warm = SV.NSServer(max_steps=512)
+    warm.run([SV.SolveRequest(N=r.N, steps=20, theta=r.theta, nu=r.nu, dt=r.dt, u0=r.u0,
+                              want_grad=r.want_grad) for r in reqs[:12]], realtime=False)
+    srv = SV.NSServer(max_steps=512)
+    res = srv.run(reqs, realtime=True)
+    live = SV.latency_stats(res)
+    srv2 = SV.NSServer(max_steps=512)
+    reqs2 = make()
+    torch.cuda.synchronize()
+    t_sat, res2 = _timed(lambda: srv2.run(reqs2, realtime=False))
+    # plain per-request path: one solver call per request, back to back
+    engines = {}
+
+    def plain():
+        out = []
+        for r in reqs2:
+            eng = engines.get(r.N)
+            if eng is None:
+                eng = engines[r.N] = E.RolloutEngine(r.N, L, 1)
+            if r.want_grad:
+                out.append(eng.rollout_and_grad(r.u0, r.nu, r.dt, r.theta, r.steps, 25))
+            else:
+                out.append(eng.rollout(r.u0, r.nu, r.dt, r.theta, r.steps))
+        return out
+    plain_small = [r for r in reqs2[:8]]
+    for r in plain_small:
+        eng = engines.get(r.N) or engines.setdefault(r.N, E.RolloutEngine(r.N, L, 1))
+        eng.rollout(r.u0, r.nu, r.dt, r.theta, 20)
+    t_plain, outs = _timed(plain)
+    byid = {r.rid: r for r in res2}
+    same = all(float(byid[r.rid].J) == float(o[0][0]) for r, o in zip(reqs2, outs))
+    return {"workload": f"{n} requests, Poisson {rate:g}/s, 30% with gradient, N in 128..2048, steps 50..500",
+            "p50_ms": live["p50_ms"], "p99_ms": live["p99_ms"], "solves_per_s_at_rate": live["solves_per_s"],
+            "saturated_solves_per_s": n / t_sat, "plain_solves_per_s": n / t_plain,
+            "speedup_vs_plain": t_plain / t_sat, "J_identical_to_plain": bool(same)}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -171,6 +292,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(u0, theta, N, L, nu, dt, args.scheme, args.cpu_seconds)
+    if not args.no_extras:
+        for key, fn in (("batched", leg_batched), ("gradient", leg_gradient), ("serving", leg_serving)):
+            try:
+                line[key] = fn()
+            except Exception as e:  # a failing extra leg must not hide the main line
+                line[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line), flush=True)
 
 

@@ -138,8 +138,9 @@ int nsk_row_pass(const nsk_plan *p, const void *spec_in, void *spec_out,
                  double *jpart, int jcap, void *stream);
 
 /* ---- rollout (SingleCoreNSSolver.rollout, NS_solver.py:732-750) ------- */
-/* Runs `steps` Chorin steps for every slot from u0 (slots[b].step must be
- * 0 on entry and is advanced by `steps`).  J_out[b] (double, device)
+/* Runs the Chorin steps of every slot from u0: slot b takes
+ * slots[b].nsteps <= steps steps (its final state is written when it gets
+ * there); slots[b].step must be 0 on entry and is advanced by `steps`.  J_out[b] (double, device)
  * receives the objective.  spec0/spec1: two spectrum buffers.
  * jpart: nsk_partials_per_slot(p, steps) doubles per slot.
  * ckpt (nullable): stores u_n for n % ckpt_every == 0, n < steps, as

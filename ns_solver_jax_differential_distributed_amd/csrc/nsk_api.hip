@@ -651,6 +651,10 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
   const long long NN2 = 2LL * p->N * p->N;
   const int ncheck = (steps + ckpt_every - 1) / ckpt_every;
   // lambda_steps: the terminal seed (row spectrum of dL/du_final) or 0
+  // Slots may have fewer steps than `steps` (the longest): slot b joins the
+  // reverse sweep at n = nsteps_b - 1 with lambda = 0 (both lambda buffers
+  // start at zero and inactive slots are skipped, so they stay zero).
+  HIP_TRY(zero_spec(p, spec3, st));
   if (lam_final) {
     RowArgs r = row_args(p, nullptr);
     r.src_real = lam_final;
@@ -698,12 +702,12 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
     // adjoint sweep over the segment
     HIP_TRY(set_step(slots, p->batch, 0, st));
     for (int n = s1 - 1; n >= s0; --n) {
-      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, 0, REQ_ANY), p->batch, st));
+      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, n, REQ_ACTIVE), p->batch, st));
       RowArgs a = row_args(p, slots);
       a.src_spec = (const float4*)lam;
       a.dst_spec = (float4*)lam_next;
       a.step_off = n;
-      a.require = REQ_ANY;
+      a.require = REQ_ACTIVE;
       a.part = gpart;
       a.pcap = steps;
       a.objective = objective;

@@ -86,10 +86,20 @@ class RolloutEngine:
     def bands(self) -> int:
         return int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(self.plan), 1))
 
-    def slots(self, nu, dt, theta, steps: int, jseed=1.0) -> torch.Tensor:
+    def step_counts(self, steps) -> np.ndarray:
+        """Per-slot step counts (an int or a (B,) array)."""
+        a = np.asarray(steps.detach().cpu().numpy() if isinstance(steps, torch.Tensor) else steps,
+                       dtype=np.int64)
+        if a.ndim == 0:
+            a = np.full(self.batch, int(a))
+        if a.shape != (self.batch,) or (a < 0).any():
+            raise ValueError(f"steps must be a non-negative int or ({self.batch},) array")
+        return a
+
+    def slots(self, nu, dt, theta, steps, jseed=1.0) -> torch.Tensor:
         th = theta_table(theta, self.batch)
         rec = _lib.slot_records(per_slot(nu, self.batch, "nu"), per_slot(dt, self.batch, "dt"),
-                                th[:, 0], th[:, 1], 0, steps, self.batch, jseed)
+                                th[:, 0], th[:, 1], 0, self.step_counts(steps), self.batch, jseed)
         t = self.buf("slots", (self.batch, _lib.SLOT_WORDS), torch.int32)
         t.copy_(torch.from_numpy(rec))
         return t
@@ -105,14 +115,14 @@ class RolloutEngine:
         return u.contiguous()
 
     # ---------------------------------------------------------------- forward
-    def rollout(self, u0, nu, dt, theta, steps: int, ckpt_every: Optional[int] = None):
-        """(J (B,) float64, u_final (B,2,N,N), ckpt or None).  With
-        ckpt_every, u_n is stored every ckpt_every steps for the adjoint."""
-        steps = int(steps)
-        if steps < 0:
-            raise ValueError("steps must be >= 0")
+    def rollout(self, u0, nu, dt, theta, steps, ckpt_every: Optional[int] = None):
+        """(J (B,) float64, u_final (B,2,N,N), ckpt or None).  ``steps`` is an
+        int or per-slot (B,) counts.  With ckpt_every, u_n is stored every
+        ckpt_every steps for the adjoint."""
+        counts = self.step_counts(steps)
+        steps = int(counts.max())
         u0 = self.field(u0)
-        slots = self.slots(nu, dt, theta, steps)
+        slots = self.slots(nu, dt, theta, counts)
         nb = self.bands()
         jpart = self.buf("jpart", (self.batch * max(steps, 1) * nb,), torch.float64)
         J = torch.empty((self.batch,), dtype=torch.float64, device=u0.device)
@@ -132,14 +142,18 @@ class RolloutEngine:
         return J, u_final, ckpt
 
     # ---------------------------------------------------------------- adjoint
-    def adjoint(self, nu, dt, theta, steps: int, ckpt: torch.Tensor, ckpt_every: int,
+    def adjoint(self, nu, dt, theta, steps, ckpt: torch.Tensor, ckpt_every: int,
                 jseed=1.0, lam_final: Optional[torch.Tensor] = None, want_du0: bool = True):
-        """Reverse sweep from checkpoints of a previous rollout.  Returns
-        (dJ/dtheta (B,2) float64, dJ/du0 (B,2,N,N) or None) for the seeds
-        jseed = dL/dJ (scalar or (B,)) and lam_final = dL/du_final."""
-        steps = int(steps)
+        """Reverse sweep from checkpoints of a previous rollout with the same
+        arguments.  Returns (dJ/dtheta (B,2) float64, dJ/du0 (B,2,N,N) or
+        None) for the seeds jseed = dL/dJ (scalar or (B,)) and lam_final =
+        dL/du_final (needs equal step counts)."""
+        counts = self.step_counts(steps)
+        steps = int(counts.max())
+        if lam_final is not None and (counts != steps).any():
+            raise ValueError("a terminal seed needs the same step count in every slot")
         k = max(1, min(int(ckpt_every), steps))
-        slots = self.slots(nu, dt, theta, steps, per_slot(jseed, self.batch, "jseed"))
+        slots = self.slots(nu, dt, theta, counts, per_slot(jseed, self.batch, "jseed"))
         # the adjoint expects slots[b].step == steps on entry
         slots[:, 4] = steps
         nb = self.bands()
@@ -157,10 +171,9 @@ class RolloutEngine:
             seg.data_ptr(), k, _lib.stream_ptr()), "rollout_adjoint")
         return dth, du0
 
-    def rollout_and_grad(self, u0, nu, dt, theta, steps: int, ckpt_every: int = 50):
-        """(J, u_final, dJ/dtheta (B,2), dJ/du0)."""
-        steps = int(steps)
-        if steps < 1:
+    def rollout_and_grad(self, u0, nu, dt, theta, steps, ckpt_every: int = 50):
+        """(J, u_final, dJ/dtheta (B,2), dJ/du0); ``steps`` int or (B,)."""
+        if int(self.step_counts(steps).max()) < 1:
             u0 = self.field(u0)
             z = torch.zeros((self.batch,), dtype=torch.float64, device=u0.device)
             return z, u0.clone(), torch.zeros((self.batch, 2), dtype=torch.float64,

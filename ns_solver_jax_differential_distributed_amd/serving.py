@@ -1,0 +1,442 @@
+"""Continuous-batching server for rollout and rollout+gradient solves.
+
+Requests of mixed grid size N, step count and parameters arrive over time.
+State stays on the GPU from admission to completion; the host only enqueues
+work and harvests results.
+
+* Forward requests (rollout only) are served by one bucket per N: a fixed
+  table of C slots with their own spectra, final states and objective
+  partial sums.  Between chunks of ``chunk`` steps, waiting requests are
+  admitted into free slots (``nsk_admit``: first predictor from u0 straight
+  into the slot) and every bucket advances all its slots by one chunk
+  (``nsk_advance``, a HIP-graph replay; each slot is masked by its own
+  step count, and a slot that reaches its last step writes its final state
+  in that chunk).  Requests therefore join and leave a running batch
+  without waiting for each other.
+* Gradient requests are grouped per N (up to ``grad_group`` of them that
+  are waiting when the lane is free) and run as one batched forward +
+  discrete adjoint with per-slot step counts (``nsk_rollout`` with
+  checkpoints, then ``nsk_rollout_adjoint``).
+* Every bucket and gradient lane has its own HIP stream (all at the default
+  priority), so work of different sizes overlaps on the device.
+* Admission is bounded by an HBM budget: bucket tables and gradient
+  workspaces are only allocated while the running total stays under it;
+  requests that do not fit wait.
+
+Each slot's arithmetic is independent of the other slots, so a request's
+result is bit-identical whether it runs alone or in a mix.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import RolloutEngine, check_n
+
+
+@dataclass
+class SolveRequest:
+    """One solve: N x N grid on [0, L)^2, ``steps`` Chorin steps."""
+    N: int
+    steps: int
+    theta: Sequence[float]
+    nu: float
+    dt: float
+    u0: Optional[object] = None      # (2, N, N); None: workloads.initial_velocity(N, seed)
+    seed: int = 0
+    want_grad: bool = False
+    arrival: float = 0.0             # seconds on the server clock
+    rid: int = -1
+
+
+@dataclass
+class SolveResult:
+    rid: int
+    N: int
+    steps: int
+    J: float
+    u_final: torch.Tensor            # (2, N, N) on the GPU
+    dtheta: Optional[np.ndarray]     # (2,) = dJ/d(theta[0], theta[1]) for gradient requests
+    du0: Optional[torch.Tensor]
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class HBMBudget:
+    def __init__(self, nbytes: int):
+        self.limit = int(nbytes)
+        self.used = 0
+
+    def take(self, nbytes: int) -> bool:
+        if self.used + nbytes > self.limit:
+            return False
+        self.used += nbytes
+        return True
+
+    def give(self, nbytes: int) -> None:
+        self.used = max(0, self.used - nbytes)
+
+
+def _u0_for(req: SolveRequest, L: float) -> torch.Tensor:
+    from . import workloads
+    if req.u0 is None:
+        u = workloads.initial_velocity(req.N, seed=req.seed, kmax=min(8, req.N // 2 - 1), L=L)
+        return torch.from_numpy(u).cuda()
+    u = req.u0 if isinstance(req.u0, torch.Tensor) else torch.as_tensor(np.asarray(req.u0))
+    return u.to(device="cuda", dtype=torch.float32).contiguous()
+
+
+class _Bucket:
+    """C forward slots of one grid size, advanced together."""
+
+    def __init__(self, srv: "NSServer", N: int, capacity: int):
+        self.srv, self.N, self.cap = srv, N, capacity
+        L = srv.L
+        mu0, sigma0, C = srv.phys
+        self.plan = _lib.make_plan(N, L, mu0, sigma0, C, capacity)
+        self.plan1 = _lib.make_plan(N, L, mu0, sigma0, C, 1)
+        self.nb = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(self.plan1), 1))
+        self.jcap = srv.max_steps
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.Stream()
+        with torch.cuda.stream(self.stream):
+            self.spec_a = torch.zeros((capacity, N, N // 2, 4), dtype=torch.float32, device=dev)
+            self.spec_b = torch.zeros_like(self.spec_a)
+            self.ufinal = torch.zeros((capacity, 2, N, N), dtype=torch.float32, device=dev)
+            self.jpart = torch.zeros((capacity, self.jcap * self.nb), dtype=torch.float64, device=dev)
+            self.J = torch.zeros((capacity,), dtype=torch.float64, device=dev)
+            rec = _lib.slot_records(0.0, 0.0, 0.0, 0.0, 0, 0, capacity)
+            self.slots = torch.from_numpy(rec).to(dev)
+        self.req: List[Optional[SolveRequest]] = [None] * capacity
+        self.step = np.zeros(capacity, dtype=np.int64)
+        self.admit_t = np.zeros(capacity)
+        self.pending_done = []          # (event, [(slot, ...)])
+        self.last_event: Optional[torch.cuda.Event] = None
+
+    @staticmethod
+    def slot_bytes(N: int, jcap: int, nb: int) -> int:
+        return 3 * 8 * N * N + 8 * jcap * nb + 64
+
+    def free(self) -> List[int]:
+        return [b for b in range(self.cap) if self.req[b] is None]
+
+    def active(self) -> bool:
+        return any(r is not None for r in self.req)
+
+    def admit(self, req: SolveRequest, b: int, now: float) -> None:
+        L = self.srv.L
+        lib = _lib.lib()
+        with torch.cuda.stream(self.stream):
+            u0 = _u0_for(req, L)
+            rec = _lib.slot_records(req.nu, req.dt, float(req.theta[0]), float(req.theta[1]), 0,
+                                    req.steps, 1)
+            self.slots[b].copy_(torch.from_numpy(rec[0]), non_blocking=False)
+            st = self.stream.cuda_stream
+            _lib.check(lib.nsk_admit(ctypes.byref(self.plan1), u0.data_ptr(), self.slots[b].data_ptr(),
+                                     _lib.SCHEME[self.srv.scheme], _lib.OBJECTIVE[self.srv.objective],
+                                     self.spec_a[b].data_ptr(), self.jpart[b].data_ptr(), self.jcap, st),
+                       "admit")
+            u0.record_stream(self.stream)
+        self.req[b] = req
+        self.step[b] = 0
+        self.admit_t[b] = now
+
+    def advance(self) -> List[int]:
+        """One chunk for every slot; returns the slots that finished in it."""
+        lib = _lib.lib()
+        k = self.srv.chunk
+        with torch.cuda.stream(self.stream):
+            _lib.check(lib.nsk_advance(ctypes.byref(self.plan), self.slots.data_ptr(), k,
+                                       _lib.SCHEME[self.srv.scheme], _lib.OBJECTIVE[self.srv.objective],
+                                       self.spec_a.data_ptr(), self.spec_b.data_ptr(), self.ufinal.data_ptr(),
+                                       self.jpart.data_ptr(), self.jcap, int(self.srv.use_graph),
+                                       self.stream.cuda_stream), "advance")
+        done = []
+        for b in range(self.cap):
+            r = self.req[b]
+            if r is None:
+                continue
+            before = self.step[b]
+            self.step[b] += k
+            if before < r.steps <= self.step[b]:
+                done.append(b)
+        if done:
+            lib2 = _lib.lib()
+            with torch.cuda.stream(self.stream):
+                outs = []
+                for b in done:
+                    _lib.check(lib2.nsk_finalize(ctypes.byref(self.plan1), self.slots[b].data_ptr(),
+                                                 self.jpart[b].data_ptr(), self.jcap, 1, self.J[b:b + 1].data_ptr(),
+                                                 self.stream.cuda_stream), "finalize")
+                    outs.append((b, self.ufinal[b].clone(), self.J[b:b + 1].clone()))
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+            self.pending_done.append((ev, outs))
+        self.last_event = torch.cuda.Event()
+        self.last_event.record(self.stream)
+        return done
+
+    def harvest(self, now_fn, results: List[SolveResult], block: bool) -> None:
+        keep = []
+        for ev, outs in self.pending_done:
+            if block:
+                ev.synchronize()
+            if ev.query():
+                t = now_fn()
+                for b, uf, J in outs:
+                    r = self.req[b]
+                    results.append(SolveResult(r.rid, r.N, r.steps, float(J.item()), uf, None, None,
+                                               r.arrival, self.admit_t[b], t))
+                    self.req[b] = None
+            else:
+                keep.append((ev, outs))
+        self.pending_done = keep
+
+
+class _GradLane:
+    """Batched forward + adjoint for gradient requests of one grid size."""
+
+    def __init__(self, srv: "NSServer", N: int):
+        self.srv, self.N = srv, N
+        self.stream = torch.cuda.Stream()
+        self.engines: Dict[int, RolloutEngine] = {}
+        self.running = None      # (event, reqs, outputs, bytes, admitted)
+
+    def busy(self) -> bool:
+        return self.running is not None
+
+    @staticmethod
+    def group_bytes(N: int, B: int, steps: int, k: int) -> int:
+        ncheck = (steps + k - 1) // k
+        return B * (8 * N * N * (4 + 1 + 1) + 8 * N * N * (ncheck + k) + 16 * steps * 256) + 1024
+
+    def launch(self, reqs: List[SolveRequest], now: float, nbytes: int) -> None:
+        B = len(reqs)
+        eng = self.engines.get(B)
+        srv = self.srv
+        if eng is None:
+            mu0, sigma0, C = srv.phys
+            eng = RolloutEngine(self.N, srv.L, B, mu0, sigma0, C, srv.scheme, srv.objective, False)
+            self.engines[B] = eng
+        with torch.cuda.stream(self.stream):
+            u0 = torch.stack([_u0_for(r, srv.L) for r in reqs])
+            th = np.array([[float(r.theta[0]), float(r.theta[1])] for r in reqs], np.float32)
+            nu = np.array([r.nu for r in reqs])
+            dt = np.array([r.dt for r in reqs])
+            steps = np.array([r.steps for r in reqs])
+            J, uf, dth, du0 = eng.rollout_and_grad(u0, nu, dt, th, steps, srv.grad_ckpt_every)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self.running = (ev, reqs, (J, uf, dth, du0), nbytes, now)
+
+    def harvest(self, now_fn, results: List[SolveResult], block: bool) -> int:
+        if self.running is None:
+            return 0
+        ev, reqs, (J, uf, dth, du0), nbytes, t_adm = self.running
+        if block:
+            ev.synchronize()
+        if not ev.query():
+            return 0
+        t = now_fn()
+        Jh, dh = J.cpu().numpy(), dth.cpu().numpy()
+        for i, r in enumerate(reqs):
+            results.append(SolveResult(r.rid, r.N, r.steps, float(Jh[i]), uf[i], dh[i].copy(), du0[i],
+                                       r.arrival, t_adm, t))
+        self.running = None
+        return nbytes
+
+
+class NSServer:
+    """Continuous-batching rollout / rollout+gradient server on one GPU.
+
+    L, phys: domain length and (mu0, sigma0, C) shared by all requests;
+    capacity: slots per forward bucket, per N; chunk: steps per bucket
+    launch (even); hbm_budget: bytes the server may allocate."""
+
+    DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 48, 1024: 16, 2048: 6, 4096: 2}
+
+    def __init__(self, L: float = 2 * math.pi, phys=(0.5, 0.1, 1.0), scheme: str = "central",
+                 objective: str = "work", chunk: int = 16, max_steps: int = 512,
+                 capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 64 << 30,
+                 grad_group: int = 8, grad_ckpt_every: int = 25, use_graph: bool = True):
+        if chunk < 2 or chunk % 2:
+            raise ValueError("chunk must be an even number >= 2")
+        if scheme not in _lib.SCHEME or objective not in _lib.OBJECTIVE:
+            raise ValueError("bad scheme or objective")
+        self.L, self.phys = float(L), tuple(float(v) for v in phys)
+        self.scheme, self.objective = scheme, objective
+        self.chunk, self.max_steps = int(chunk), int(max_steps)
+        self.capacity = dict(self.DEFAULT_CAPACITY if capacity is None else capacity)
+        self.budget = HBMBudget(hbm_budget)
+        self.grad_group, self.grad_ckpt_every = int(grad_group), int(grad_ckpt_every)
+        self.use_graph = bool(use_graph)
+        self.buckets: Dict[int, _Bucket] = {}
+        self.lanes: Dict[int, _GradLane] = {}
+        self.queue: deque = deque()
+        self.gqueue: Dict[int, deque] = {}
+        self.results: List[SolveResult] = []
+        self._rid = 0
+        self.t0 = time.perf_counter()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    # ---------------------------------------------------------------- requests
+    def submit(self, req: SolveRequest) -> int:
+        check_n(req.N)
+        if req.steps < 1 or req.steps > self.max_steps:
+            raise ValueError(f"steps must be in [1, {self.max_steps}], got {req.steps}")
+        if len(req.theta) < 2:
+            raise ValueError("theta needs at least 2 entries")
+        if req.rid < 0:
+            req.rid = self._rid
+            self._rid += 1
+        if req.arrival <= 0:
+            req.arrival = self.now()
+        if req.want_grad:
+            self.gqueue.setdefault(req.N, deque()).append(req)
+        else:
+            self.queue.append(req)
+        return req.rid
+
+    def pending(self) -> int:
+        n = len(self.queue) + sum(len(q) for q in self.gqueue.values())
+        n += sum(sum(r is not None for r in b.req) for b in self.buckets.values())
+        n += sum(len(l.running[1]) for l in self.lanes.values() if l.running)
+        return n
+
+    # ---------------------------------------------------------------- engine loop
+    def _bucket(self, N: int) -> Optional[_Bucket]:
+        b = self.buckets.get(N)
+        if b is not None:
+            return b
+        cap = self.capacity.get(N, 4)
+        nb = 256 if N >= 2048 else max(1, N // 16)
+        per = _Bucket.slot_bytes(N, self.max_steps, nb)
+        while cap > 0 and not self.budget.take(per * cap):
+            cap //= 2
+        if cap == 0:
+            return None
+        b = _Bucket(self, N, cap)
+        self.buckets[N] = b
+        return b
+
+    def _admit(self) -> None:
+        now = self.now()
+        waiting = deque()
+        free = {}
+        while self.queue:
+            r = self.queue.popleft()
+            bk = self._bucket(r.N)
+            if bk is None:
+                waiting.append(r)
+                continue
+            fl = free.setdefault(r.N, bk.free())
+            if not fl:
+                waiting.append(r)
+                continue
+            bk.admit(r, fl.pop(0), now)
+        self.queue = waiting
+        for N, q in self.gqueue.items():
+            if not q:
+                continue
+            lane = self.lanes.setdefault(N, _GradLane(self, N))
+            if lane.busy():
+                continue
+            grp = [q.popleft() for _ in range(min(self.grad_group, len(q)))]
+            need = _GradLane.group_bytes(N, len(grp), max(r.steps for r in grp), self.grad_ckpt_every)
+            if not self.budget.take(need):
+                for r in reversed(grp):
+                    q.appendleft(r)
+                continue
+            lane.launch(grp, now, need)
+
+    def step(self) -> None:
+        """Admit waiting requests, advance every busy bucket by one chunk
+        (each on its own stream), wait for those chunks (not for running
+        gradient groups) and harvest whatever has finished."""
+        self._admit()
+        launched = []
+        for bk in self.buckets.values():
+            if bk.active():
+                bk.advance()
+                launched.append(bk)
+        for bk in launched:
+            bk.last_event.synchronize()
+        if not launched:
+            # only gradient groups in flight: wait for the first to finish
+            busy = [l for l in self.lanes.values() if l.busy()]
+            if busy and not self.queue:
+                busy[0].running[0].synchronize()
+        for bk in self.buckets.values():
+            bk.harvest(self.now, self.results, False)
+        for lane in self.lanes.values():
+            self.budget.give(lane.harvest(self.now, self.results, False))
+
+    def run(self, requests: Sequence[SolveRequest], realtime: bool = True) -> List[SolveResult]:
+        """Serve requests (their ``arrival`` fields are offsets from the start
+        when realtime) until all are done; returns the results in completion
+        order."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        start = len(self.results)
+        i = 0
+        while i < len(reqs) or self.pending():
+            now = self.now()
+            while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
+                r = reqs[i]
+                if not realtime:
+                    r.arrival = now
+                self.submit(r)
+                i += 1
+            if self.pending():
+                self.step()
+            else:
+                time.sleep(min(1e-3, max(0.0, reqs[i].arrival - now)) if i < len(reqs) else 0)
+        return self.results[start:]
+
+
+def latency_stats(results: Sequence[SolveResult]) -> dict:
+    lat = np.array([r.latency for r in results])
+    span = max(r.finished for r in results) - min(r.arrival for r in results)
+    return {"count": len(results), "solves_per_s": len(results) / span if span > 0 else float("inf"),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "makespan_s": span}
+
+
+def poisson_requests(n: int, rate: float, seed: int = 0, grad_frac: float = 0.3,
+                     sizes=(128, 256, 512, 1024, 2048), weights=(0.3, 0.3, 0.2, 0.15, 0.05),
+                     steps_range=(50, 500), L: float = 2 * math.pi) -> List[SolveRequest]:
+    """The served workload: Poisson arrivals at ``rate`` per second; N drawn
+    with ``weights``; steps uniform in steps_range; seeded u0 (Taylor-Green
+    + perturbation, max|u0| <= 1.2), theta ~ U[0.5,1.5]^3, nu log-uniform
+    [1e-4, 1e-2], dt at CFL 0.3 capped by the diffusive limit and by the
+    stability limit of central advection, 0.9 * 2 nu / 1.2^2."""
+    from . import workloads
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        N = int(rng.choice(sizes, p=np.asarray(weights) / np.sum(weights)))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        th = rng.uniform(0.5, 1.5, 3).astype(np.float32)
+        nu = float(np.exp(rng.uniform(math.log(1e-4), math.log(1e-2))))
+        dx = L / N
+        dt = float(np.float32(min(0.3 * dx / 1.2, dx * dx / (4 * nu), 0.9 * 2 * nu / 1.44)))
+        out.append(SolveRequest(N=N, steps=steps, theta=th, nu=nu, dt=dt, seed=seed * 100003 + i,
+                                want_grad=bool(rng.random() < grad_frac), arrival=t))
+    return out

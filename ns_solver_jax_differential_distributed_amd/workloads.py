@@ -62,11 +62,19 @@ def nu_batch(n: int, seed: int = 0, lo: float = 1e-4, hi: float = 1e-2) -> np.nd
     return np.exp(r).astype(np.float32)
 
 
-def cfl_dt(u0: np.ndarray, L: float, cfl: float = 0.3, nu: float | None = None) -> float:
-    """dt = cfl*dx/max|u0|, also <= dx^2/(4 nu) when nu is given."""
+def cfl_dt(u0: np.ndarray, L: float, cfl: float = 0.3, nu: float | None = None,
+           stable: bool = False) -> float:
+    """dt = cfl*dx/max|u0|, also <= dx^2/(4 nu) when nu is given.  With
+    stable=True also dt <= 0.9 * 2 nu / max|u0|^2: explicit Euler with central
+    advection amplifies every mode unless the diffusion keeps up (von Neumann:
+    (u dt/dx)^2 <= 2 nu dt/dx^2), and trajectories whose results are compared
+    against each other must stay finite."""
     N = u0.shape[-1]
     dx = L / N
-    dt = cfl * dx / float(np.abs(u0).max())
+    umax = float(np.abs(u0).max())
+    dt = cfl * dx / umax
     if nu is not None:
         dt = min(dt, dx * dx / (4.0 * nu))
+        if stable:
+            dt = min(dt, 0.9 * 2.0 * nu / (umax * umax))
     return float(np.float32(dt))

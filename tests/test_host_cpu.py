@@ -238,3 +238,18 @@ def test_layout_model_round_trip_projection_poisson(N):
     r = rng.standard_normal((N, N))
     p = _row_inv(_col(_row_fwd(np.stack([r, 0 * r])), L, "poisson"))[0]
     np.testing.assert_allclose(p, O.poisson(r, L / N), atol=1e-12)
+
+
+def test_hbm_budget_accounting():
+    from ns_solver_jax_differential_distributed_amd.serving import HBMBudget, poisson_requests
+    b = HBMBudget(100)
+    assert b.take(60) and not b.take(50) and b.take(40)
+    b.give(60)
+    assert b.used == 40 and b.take(60)
+    reqs = poisson_requests(200, rate=200.0, seed=3)
+    Ns = [r.N for r in reqs]
+    assert set(Ns) <= {128, 256, 512, 1024, 2048}
+    assert all(50 <= r.steps <= 500 for r in reqs)
+    assert 0.15 < np.mean([r.want_grad for r in reqs]) < 0.45
+    arr = np.diff([0.0] + [r.arrival for r in reqs])
+    assert 0.5 / 200 < arr.mean() < 2.0 / 200
