@@ -47,7 +47,22 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched / gradient / serving legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 10 device copies of 256 MiB (scale for HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(nbytes=256 << 20, reps=10):
+    """A plain device copy of known size: what the FETCH_SIZE / WRITE_SIZE
+    counters read for it fixes their scale (each copy reads and writes
+    nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def cpu_baseline(u0, theta, N, L, nu, dt, scheme, budget_s):
@@ -233,6 +248,9 @@ def leg_serving(n=300, rate=200.0):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
