@@ -13,10 +13,11 @@
  *    exactly the reference's (2, N, N) arrays; a batch of B independent
  *    fields is B such blocks back to back.
  *  - "Spectra" are the row-spectral form of a pair of real fields (a, b):
- *    per row x, the half spectrum along y of a and of b, stored as
- *    float4 {Re A, Im A, Re B, Im B} for ky = 1..N/2-1, with ky = 0 holding
- *    {A(0), A(N/2), B(0), B(N/2)} (both real).  One spectrum buffer is
- *    N*N/2 float4 = 8*N*N bytes per field pair.
+ *    per row x, Z = FFT_y(a + i b) (N complex values), stored as float4
+ *    {Re Z(c), Im Z(c), Re Z(N-c), Im Z(N-c)} at c = 1..N/2-1 and
+ *    {Re Z(0), Im Z(0), Re Z(N/2), Im Z(N/2)} at c = 0.  Between rollout
+ *    steps Z carries the factor 1/N of the inverse row transform.  One
+ *    spectrum buffer is N*N/2 float4 = 8*N*N bytes per field pair.
  *  - N must be a power of two with 8 <= N <= 4096 (NSK_E_UNSUPPORTED_N
  *    otherwise).
  *  - Work is enqueued on the given stream; nothing synchronises.

@@ -1,36 +1,51 @@
-// nsk_fft.h - in-LDS complex FFT for lines of N = 2^m points (8 <= N <= 4096)
-// on CDNA4 (gfx950), used by the row and column passes of the spectral
-// pressure solve.
+// nsk_fft.h - complex FFT of lines of N = 2^m points (8 <= N <= 4096) held
+// in registers, for CDNA4 (gfx950).
 //
-// Algorithm: Stockham autosort FFT with mixed radices 16/8/4.  Each stage
-// reads R points with stride N/R from one LDS line, applies the inter-stage
-// twiddles, does an R-point DFT in registers and writes the R results back
-// to positions (i-k)*R + k + j*P of the same line.  All loads of a stage
-// complete before a workgroup barrier and all stores happen after it, so
-// one LDS line per transform suffices.  Lines are padded by one element
-// per 16 (pad()) to spread the strided accesses of the first stages over
-// the LDS banks.
+// A line is held by T = N/EE threads, EE = min(E, N) points each (E = 8 or
+// 16): thread t owns points t + T*m, m = 0..EE-1, on entry and the
+// transform at the same indices on exit.  The transform is a Stockham
+// autosort FFT with radices R_0 = EE, R_1, ..., R_(S-1) (each <= EE):
+//
+//   pass s (radix R, P = R_0 * ... * R_(s-1)): butterfly b < N/R reads the
+//   points b + j*N/R (j < R), multiplies point j by w^(j*(b mod P)),
+//   w = exp(DIR*2*pi*i/(P*R)), takes the R-point DFT and writes output j to
+//   position (b - b mod P)*R + b mod P + j*P.
+//
+// Thread t does butterflies b = t + T*q, q < EE/R.  Their inputs are the
+// points t + T*(q + j*EE/R) - the thread's own registers when the pass
+// input has the "t + T*m" distribution.  The first pass therefore needs no
+// data movement, and the outputs of the last pass (P*R = N, so b < P and
+// output j lands on b + j*P) are again the thread's own points.  Between
+// passes the data goes through LDS once: write at the Stockham positions,
+// barrier, read points t + T*m.  4096 points with E = 16 take two such
+// exchanges (radices 16, 16, 16).
+//
+// LDS positions are padded by one slot per 16 (pad(m) = m + m/16): the
+// strided writes of a pass (stride 16 or 17 slots between lanes) and the
+// unit-stride reads then hit distinct banks.
 //
 // Twiddles come from a table of exp(-2 pi i m / 4096) computed in double
-// precision on the device (g_tw); an N-point transform reads every
-// (4096/N)-th entry.
+// precision on the device (g_tw, one copy per translation unit).  A
+// thread's twiddles depend only on t; they are fetched once per kernel
+// (Twid): into registers for the passes with many distinct values, into a
+// small LDS table for the others.
 #pragma once
 #include <hip/hip_runtime.h>
+
+// Keeps the scheduler from interleaving independent lines (and so from
+// holding the working sets of several transforms in registers at once).
+#ifndef NSK_NO_SCHED_FENCE
+#define NSK_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define NSK_SCHED_FENCE() ((void)0)
+#endif
 
 namespace nsk {
 
 constexpr int TW_N = 4096;
-// One table per translation unit (internal linkage); filled by k_init_tw.
 namespace {
 __device__ float2 g_tw[TW_N];
 }
-
-__device__ __forceinline__ int pad(int m) { return m + (m >> 4); }
-
-template <int N>
-struct LineLen {
-  static constexpr int NP = N + N / 16;
-};
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -38,27 +53,30 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// cos / sin of 2*pi*i/16, i = 0..7
-__device__ __forceinline__ constexpr float w16c(int i) {
-  return i == 0 ? 1.0f
-       : i == 1 ? 0.923879532511286756f
-       : i == 2 ? 0.707106781186547524f
-       : i == 3 ? 0.382683432365089772f
-       : i == 4 ? 0.0f
-       : i == 5 ? -0.382683432365089772f
-       : i == 6 ? -0.707106781186547524f
-                : -0.923879532511286756f;
+// cos / sin of 2*pi*i/16, i = 0..15 (non-recursive so that it folds to a
+// constant wherever i does)
+__host__ __device__ __forceinline__ constexpr float w16c(int i) {
+  constexpr float C1 = 0.923879532511286756f, C2 = 0.707106781186547524f, C3 = 0.382683432365089772f;
+  switch (i & 15) {
+    case 0: return 1.0f;
+    case 1: return C1;
+    case 2: return C2;
+    case 3: return C3;
+    case 4: return 0.0f;
+    case 5: return -C3;
+    case 6: return -C2;
+    case 7: return -C1;
+    case 8: return -1.0f;
+    case 9: return -C1;
+    case 10: return -C2;
+    case 11: return -C3;
+    case 12: return 0.0f;
+    case 13: return C3;
+    case 14: return C2;
+    default: return C1;
+  }
 }
-__device__ __forceinline__ constexpr float w16s(int i) {
-  return i == 0 ? 0.0f
-       : i == 1 ? 0.382683432365089772f
-       : i == 2 ? 0.707106781186547524f
-       : i == 3 ? 0.923879532511286756f
-       : i == 4 ? 1.0f
-       : i == 5 ? 0.923879532511286756f
-       : i == 6 ? 0.707106781186547524f
-                : 0.382683432365089772f;
-}
+__host__ __device__ __forceinline__ constexpr float w16s(int i) { return w16c((i + 12) & 15); }
 
 // v * exp(DIR * 2 pi i q / R) for a compile-time q (after unrolling).
 template <int R, int DIR>
@@ -71,8 +89,8 @@ __device__ __forceinline__ float2 rot_const(int q, float2 v) {
   return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
 }
 
-// In-register R-point DFT, natural order in and out:
-// x[q] <- sum_j x[j] exp(DIR * 2 pi i j q / R).  Radix-2 decimation in time.
+// In-register R-point DFT, natural order in and out (radix-2 decimation in
+// time): x[q] <- sum_j x[j] exp(DIR * 2 pi i j q / R).
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2 (&x)[R]) {
   if constexpr (R == 1) {
@@ -99,134 +117,227 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
   }
 }
 
-// Radix plan per transform length: up to three Stockham stages.
-template <int N> struct Radix;
-template <> struct Radix<4096> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 16; };
-template <> struct Radix<2048> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 8; };
-template <> struct Radix<1024> { static constexpr int S = 3, R0 = 16, R1 = 16, R2 = 4; };
-template <> struct Radix<512> { static constexpr int S = 3, R0 = 8, R1 = 8, R2 = 8; };
-template <> struct Radix<256> { static constexpr int S = 2, R0 = 16, R1 = 16, R2 = 1; };
-template <> struct Radix<128> { static constexpr int S = 2, R0 = 16, R1 = 8, R2 = 1; };
-template <> struct Radix<64> { static constexpr int S = 2, R0 = 8, R1 = 8, R2 = 1; };
-template <> struct Radix<32> { static constexpr int S = 2, R0 = 8, R1 = 4, R2 = 1; };
-template <> struct Radix<16> { static constexpr int S = 1, R0 = 16, R1 = 1, R2 = 1; };
-template <> struct Radix<8> { static constexpr int S = 1, R0 = 8, R1 = 1, R2 = 1; };
+__host__ __device__ __forceinline__ constexpr int pad(int m) { return m + (m >> 4); }
+template <int N>
+struct LineLen {
+  static constexpr int NP = N + N / 16;
+};
 
-// Per-thread twiddles of the second and third stages for a fixed mapping
-// of (NL lines, T threads) to butterflies.  Loaded once per kernel when a
-// workgroup transforms many line sets with the same mapping; the inverse
-// transform uses the conjugates.
-template <int N, int NL, int T>
-struct TwCache {
-  using Rd = Radix<N>;
-  static constexpr int P1 = Rd::R0, P2 = Rd::R0 * Rd::R1;
-  static constexpr int B1 = Rd::S > 1 ? (NL * (N / Rd::R1) + T - 1) / T : 1;
-  static constexpr int B2 = Rd::S > 2 ? (NL * (N / Rd::R2) + T - 1) / T : 1;
-  static constexpr int C1 = Rd::S > 1 ? Rd::R1 - 1 : 1;
-  static constexpr int C2 = Rd::S > 2 ? Rd::R2 - 1 : 1;
-  float2 w1[B1][C1];
-  float2 w2[B2][C2];
+// ---- radix plan -----------------------------------------------------------
+constexpr int fft_e(int N, int E) { return N < E ? N : E; }
+// radix of pass s (1 past the last pass)
+constexpr int fft_radix(int N, int E, int s) {
+  const int e = fft_e(N, E);
+  int m = N;
+  for (int i = 0; i < s; ++i) m /= (m < e ? m : e);
+  return m <= 1 ? 1 : (m < e ? m : e);
+}
+constexpr int fft_passes(int N, int E) {
+  int s = 0;
+  while (fft_radix(N, E, s) > 1) ++s;
+  return s;
+}
+constexpr int fft_p(int N, int E, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= fft_radix(N, E, i);
+  return p;
+}
+constexpr int fft_ntw(int N, int E, int s) {
+  return s == 0 ? 0 : (fft_e(N, E) / fft_radix(N, E, s)) * (fft_radix(N, E, s) - 1);
+}
+constexpr int fft_twoff(int N, int E, int s) {
+  int o = 0;
+  for (int i = 1; i < s; ++i) o += fft_ntw(N, E, i);
+  return o;
+}
 
-  __device__ __forceinline__ void load(int tid) {
-    if constexpr (Rd::S > 1) {
-      constexpr int NB = N / Rd::R1, TOT = NL * NB;
+// Twiddles of pass s sit in a per-workgroup LDS table when it is small
+// (P*(R+1) <= 1024 entries; row k = b mod P holds w^(j*k), j < R, padded to
+// R+1 entries so that 16 lanes with consecutive k hit distinct banks), and
+// in registers otherwise.
+constexpr bool fft_tw_lds(int N, int E, int s) {
+  return s > 0 && fft_p(N, E, s) * (fft_radix(N, E, s) + 1) <= 1024;
+}
+constexpr int fft_nreg(int N, int E, int s) { return fft_tw_lds(N, E, s) ? 0 : fft_ntw(N, E, s); }
+constexpr int fft_regoff(int N, int E, int s) {
+  int o = 0;
+  for (int i = 1; i < s; ++i) o += fft_nreg(N, E, i);
+  return o;
+}
+constexpr int fft_nlds(int N, int E, int s) {
+  return fft_tw_lds(N, E, s) ? fft_p(N, E, s) * (fft_radix(N, E, s) + 1) : 0;
+}
+constexpr int fft_ldsoff(int N, int E, int s) {
+  int o = 0;
+  for (int i = 1; i < s; ++i) o += fft_nlds(N, E, i);
+  return o;
+}
+
+template <int N, int E>
+struct RF {
+  static constexpr int EE = fft_e(N, E);
+  static constexpr int T = N / EE;
+  static constexpr int S = fft_passes(N, E);
+  static constexpr int NREG = fft_regoff(N, E, S);   // register twiddles
+  static constexpr int NLDS = fft_ldsoff(N, E, S);   // LDS table entries
+};
+
+// This thread's twiddles (forward direction): registers for the large-P
+// passes, a pointer to the workgroup's LDS table for the others.
+template <int N, int E>
+struct Twid {
+  using F = RF<N, E>;
+  float2 w[F::NREG > 0 ? F::NREG : 1];
+  const float2* tbl;
+
+  template <int s>
+  __device__ __forceinline__ void load_pass(int t) {
+    if constexpr (s > 0 && s < F::S && !fft_tw_lds(N, E, s)) {
+      constexpr int R = fft_radix(N, E, s), P = fft_p(N, E, s), Q = F::EE / R;
+      constexpr int OFF = fft_regoff(N, E, s);
 #pragma unroll
-      for (int q = 0; q < B1; ++q) {
-        const int b = tid + q * T;
-        const int k = (b % NB) & (P1 - 1);
+      for (int q = 0; q < Q; ++q) {
+        const int k = (t + F::T * q) & (P - 1);
 #pragma unroll
-        for (int j = 1; j <= C1; ++j)
-          w1[q][j - 1] = (b < TOT) ? g_tw[(j * k) * (TW_N / (P1 * Rd::R1))] : make_float2(1.f, 0.f);
+        for (int j = 1; j < R; ++j) w[OFF + q * (R - 1) + j - 1] = g_tw[(j * k) * (TW_N / (P * R))];
       }
     }
-    if constexpr (Rd::S > 2) {
-      constexpr int NB = N / Rd::R2, TOT = NL * NB;
-#pragma unroll
-      for (int q = 0; q < B2; ++q) {
-        const int b = tid + q * T;
-        const int k = (b % NB) & (P2 - 1);
-#pragma unroll
-        for (int j = 1; j <= C2; ++j)
-          w2[q][j - 1] = (b < TOT) ? g_tw[(j * k) * (TW_N / (P2 * Rd::R2))] : make_float2(1.f, 0.f);
+  }
+  template <int s>
+  __device__ __forceinline__ void fill_pass(float2* lds_tbl, int tid, int nthreads) {
+    if constexpr (s > 0 && s < F::S && fft_tw_lds(N, E, s)) {
+      constexpr int R = fft_radix(N, E, s), P = fft_p(N, E, s);
+      constexpr int OFF = fft_ldsoff(N, E, s), CNT = fft_nlds(N, E, s);
+      for (int i = tid; i < CNT; i += nthreads) {
+        const int k = i / (R + 1), j = i % (R + 1);
+        lds_tbl[OFF + i] = (j < R) ? g_tw[(j * k) * (TW_N / (P * R))] : make_float2(1.f, 0.f);
       }
+    }
+  }
+  // Fills the LDS table (all threads of the workgroup, nthreads of them;
+  // the caller synchronises before first use) and loads the registers.
+  __device__ __forceinline__ void load(int t, float2* lds_tbl, int tid, int nthreads) {
+    tbl = lds_tbl;
+    fill_pass<1>(lds_tbl, tid, nthreads);
+    fill_pass<2>(lds_tbl, tid, nthreads);
+    fill_pass<3>(lds_tbl, tid, nthreads);
+    load_pass<1>(t);
+    load_pass<2>(t);
+    load_pass<3>(t);
+  }
+};
+
+// Pass s on this thread's points (v[m] = point t + T*m of the pass input).
+template <int N, int E, int s, int DIR>
+__device__ __forceinline__ void rpass(float2 (&v)[fft_e(N, E)], const Twid<N, E>& tw, int t) {
+  constexpr int EE = RF<N, E>::EE, T = RF<N, E>::T;
+  constexpr int R = fft_radix(N, E, s), P = fft_p(N, E, s), Q = EE / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    float2 x[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = v[q + j * Q];
+    if constexpr (P > 1) {
+      const float2* row = nullptr;
+      if constexpr (fft_tw_lds(N, E, s))
+        row = tw.tbl + fft_ldsoff(N, E, s) + ((t + T * q) & (P - 1)) * (R + 1);
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        float2 w;
+        if constexpr (fft_tw_lds(N, E, s))
+          w = row[j];
+        else
+          w = tw.w[fft_regoff(N, E, s) + q * (R - 1) + j - 1];
+        if (DIR > 0) w.y = -w.y;
+        x[j] = cmul(x[j], w);
+      }
+    }
+    dft<R, DIR>(x);
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[q + j * Q] = x[j];
+  }
+}
+
+// Writes the outputs of pass s to their Stockham positions in line L.
+template <int N, int E, int s>
+__device__ __forceinline__ void xwrite(const float2 (&v)[fft_e(N, E)], float2* L, int t) {
+  constexpr int EE = RF<N, E>::EE, T = RF<N, E>::T;
+  constexpr int R = fft_radix(N, E, s), P = fft_p(N, E, s), Q = EE / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int b = t + T * q;
+    const int k = b & (P - 1);
+    const int base = (b - k) * R + k;
+#pragma unroll
+    for (int j = 0; j < R; ++j) L[pad(base + j * P)] = v[q + j * Q];
+  }
+}
+
+// Reads points t + T*m of line L.
+template <int N, int E>
+__device__ __forceinline__ void xread(float2 (&v)[fft_e(N, E)], const float2* L, int t) {
+  constexpr int EE = RF<N, E>::EE, T = RF<N, E>::T;
+#pragma unroll
+  for (int m = 0; m < EE; ++m) v[m] = L[pad(t + T * m)];
+}
+
+// LDS exchange buffers of one thread: line l of buffer k (k = 0, 1) sits
+// at b0 + k * bstride + l * lstride (integer offsets from one LDS pointer,
+// so that the compiler keeps ds_* accesses and never spills the pointers).
+// DB (double buffering): exchange i uses buffer i % 2, so the barrier inside
+// exchange i already orders every read of exchange i-1 (other buffer)
+// before the writes of exchange i+1, and one barrier per exchange
+// suffices.  Single buffering: a barrier precedes the writes of every
+// exchange after the first.
+template <bool DB>
+struct XBuf {
+  float2* b0;
+  int bstride;
+  int lstride;
+  int k;
+  __device__ __forceinline__ float2* next() {
+    if constexpr (DB) {
+      float2* b = b0 + (k & 1) * bstride;
+      ++k;
+      return b;
+    } else {
+      if (k > 0) __syncthreads();
+      ++k;
+      return b0;
     }
   }
 };
 
-// One Stockham stage of radix R on NL lines of length N held in LDS.
-// Line l lives at buf + ((first + l) % MOD) * NP.  P = product of the radices
-// of the earlier stages.  T threads cooperate; every thread must call this.
-// tw (optional): this thread's twiddles [BPT][R-1] for the stage.
-template <int N, int R, int P, int DIR, int NL, int T, int MOD, int BPT_TW = 0, int CT = 1>
-__device__ __forceinline__ void fft_stage(float2* buf, int first, int tid,
-                                          const float2 (*tw)[CT] = nullptr) {
-  constexpr int NB = N / R;
-  constexpr int TOT = NL * NB;
-  constexpr int BPT = (TOT + T - 1) / T;
-  constexpr int NP = LineLen<N>::NP;
-  static_assert(BPT * R <= 48, "too many points per thread");
-  static_assert(BPT_TW == 0 || (BPT_TW == BPT && CT == R - 1), "twiddle cache shape");
-  float2 x[BPT][R];
+template <int N, int E, int s, int DIR, int NL, bool DB>
+__device__ __forceinline__ void rfft_from(float2 (*v)[fft_e(N, E)], XBuf<DB>& xb, int t,
+                                          const Twid<N, E>& tw) {
+  if constexpr (s < RF<N, E>::S) {
+    float2* B = xb.next();
 #pragma unroll
-  for (int q = 0; q < BPT; ++q) {
-    const int b = tid + q * T;
-    if ((TOT % T == 0) || b < TOT) {
-      const int line = b / NB, i = b % NB;
-      const float2* L = buf + ((first + line) % MOD) * NP;
+    for (int l = 0; l < NL; ++l) xwrite<N, E, s - 1>(v[l], B + l * xb.lstride, t);
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < R; ++j) x[q][j] = L[pad(i + j * NB)];
-      if constexpr (P > 1) {
-        const int k = i & (P - 1);
-#pragma unroll
-        for (int j = 1; j < R; ++j) {
-          float2 w;
-          if constexpr (BPT_TW > 0)
-            w = tw[q][j - 1];
-          else
-            w = g_tw[(j * k) * (TW_N / (P * R))];
-          if (DIR > 0) w.y = -w.y;
-          x[q][j] = cmul(x[q][j], w);
-        }
-      }
-      dft<R, DIR>(x[q]);
+    for (int l = 0; l < NL; ++l) {
+      xread<N, E>(v[l], B + l * xb.lstride, t);
+      rpass<N, E, s, DIR>(v[l], tw, t);
+      NSK_SCHED_FENCE();
     }
+    rfft_from<N, E, s + 1, DIR, NL, DB>(v, xb, t, tw);
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < BPT; ++q) {
-    const int b = tid + q * T;
-    if ((TOT % T == 0) || b < TOT) {
-      const int line = b / NB, i = b % NB;
-      const int k = i & (P - 1);
-      float2* L = buf + ((first + line) % MOD) * NP;
-      const int base = (i - k) * R + k;
-#pragma unroll
-      for (int j = 0; j < R; ++j) L[pad(base + j * P)] = x[q][j];
-    }
-  }
-  __syncthreads();
 }
 
-// Full unnormalised transform of NL lines: DIR = -1 forward, +1 inverse.
-template <int N, int DIR, int NL, int T, int MOD>
-__device__ __forceinline__ void fft_lines(float2* buf, int first, int tid) {
-  using Rd = Radix<N>;
-  fft_stage<N, Rd::R0, 1, DIR, NL, T, MOD>(buf, first, tid);
-  if constexpr (Rd::S > 1) fft_stage<N, Rd::R1, Rd::R0, DIR, NL, T, MOD>(buf, first, tid);
-  if constexpr (Rd::S > 2) fft_stage<N, Rd::R2, Rd::R0 * Rd::R1, DIR, NL, T, MOD>(buf, first, tid);
-}
-
-// Same with this thread's twiddles from a TwCache built for (NL, T).
-template <int N, int DIR, int NL, int T, int MOD>
-__device__ __forceinline__ void fft_lines_tc(float2* buf, int first, int tid,
-                                             const TwCache<N, NL, T>& tc) {
-  using Rd = Radix<N>;
-  using TC = TwCache<N, NL, T>;
-  fft_stage<N, Rd::R0, 1, DIR, NL, T, MOD>(buf, first, tid);
-  if constexpr (Rd::S > 1)
-    fft_stage<N, Rd::R1, Rd::R0, DIR, NL, T, MOD, TC::B1, TC::C1>(buf, first, tid, tc.w1);
-  if constexpr (Rd::S > 2)
-    fft_stage<N, Rd::R2, Rd::R0 * Rd::R1, DIR, NL, T, MOD, TC::B2, TC::C2>(buf, first, tid, tc.w2);
+// Unnormalised transform (DIR = -1 forward, +1 inverse) of NL lines held by
+// this thread, sharing the barriers.  Every thread of the workgroup calls
+// it (the exchanges contain __syncthreads()).
+template <int N, int E, int DIR, int NL, bool DB>
+__device__ __forceinline__ void rfft(float2 (*v)[fft_e(N, E)], XBuf<DB>& xb, int t,
+                                     const Twid<N, E>& tw) {
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    rpass<N, E, 0, DIR>(v[l], tw, t);
+    NSK_SCHED_FENCE();
+  }
+  rfft_from<N, E, 1, DIR, NL, DB>(v, xb, t, tw);
 }
 
 }  // namespace nsk

@@ -23,7 +23,7 @@ struct RowArgs {
   const float* src_real;  // [B][src_nc][N][N]
   long long src_bstride;  // floats between slots
   int src_nc;             // 1: second component is zero
-  const float4* src_spec; // [B][N][N/2]
+  const float4* src_spec; // [B][N][N/2] paired row spectra
   float* dst_real;        // [B][dst_nc][N][N]
   long long dst_bstride;
   int dst_nc;
@@ -71,58 +71,58 @@ struct StencilArgs {
   float t;
 };
 
-// Row-pass geometry for grid size N: G rows per iteration, ITER iterations
-// per workgroup (a band of G*ITER rows), T threads.
-#ifndef NSK_ROW_T4096
-#define NSK_ROW_T4096 256
+// Row-pass geometry for grid size N.  A workgroup runs LG row streams of
+// N/EE threads each (EE = min(E, N) points per thread); stream g walks ROWS
+// consecutive rows.  DB: double-buffered LDS exchanges.
+#ifndef NSK_ROW_E4096
+#define NSK_ROW_E4096 16
 #endif
-#ifndef NSK_ROW_TWR
-#define NSK_ROW_TWR 0
+#ifndef NSK_ROW_ROWS4096
+#define NSK_ROW_ROWS4096 8
+#endif
+template <int N>
+struct RowGeo;
+template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, ROWS = NSK_ROW_ROWS4096; static constexpr bool DB = true; };
+template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<256> { static constexpr int E = 16, LG = 4, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<128> { static constexpr int E = 16, LG = 8, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<64> { static constexpr int E = 16, LG = 16, ROWS = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<32> { static constexpr int E = 16, LG = 16, ROWS = 2; static constexpr bool DB = true; };
+template <> struct RowGeo<16> { static constexpr int E = 16, LG = 16, ROWS = 1; static constexpr bool DB = true; };
+template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, ROWS = 1; static constexpr bool DB = true; };
+
+// Column-pass geometry: W column pairs (lines X and Y each) per block, E
+// points per thread, NWG workgroups per slot (each loops over blocks NWG
+// apart; PF: prefetch the next block into registers).
+#ifndef NSK_COL_E4096
+#define NSK_COL_E4096 16
 #endif
 #ifndef NSK_COL_W4096
 #define NSK_COL_W4096 2
 #endif
-template <int N>
-struct RowCfg;
-template <> struct RowCfg<4096> { static constexpr int G = 2, ITER = 8, T = NSK_ROW_T4096; };
-template <> struct RowCfg<2048> { static constexpr int G = 2, ITER = 4, T = 256; };
-template <> struct RowCfg<1024> { static constexpr int G = 4, ITER = 2, T = 256; };
-template <> struct RowCfg<512> { static constexpr int G = 8, ITER = 1, T = 512; };
-template <> struct RowCfg<256> { static constexpr int G = 16, ITER = 1, T = 256; };
-template <> struct RowCfg<128> { static constexpr int G = 32, ITER = 1, T = 256; };
-template <> struct RowCfg<64> { static constexpr int G = 64, ITER = 1, T = 512; };
-template <> struct RowCfg<32> { static constexpr int G = 32, ITER = 1, T = 256; };
-template <> struct RowCfg<16> { static constexpr int G = 16, ITER = 1, T = 256; };
-template <> struct RowCfg<8> { static constexpr int G = 8, ITER = 1, T = 256; };
-
-// Column-pass geometry: W spectral columns (two fields each) per block, T
-// threads, NWG workgroups per slot (each loops over blocks NWG apart).
-template <int N>
-struct ColCfg;
-#ifndef NSK_COL_T4096
-#define NSK_COL_T4096 512
-#endif
 #ifndef NSK_COL_NWG4096
 #define NSK_COL_NWG4096 1024
 #endif
-#ifndef NSK_COL_WPE
-#define NSK_COL_WPE 0
+#ifndef NSK_COL_PF
+#define NSK_COL_PF 0
 #endif
-template <> struct ColCfg<4096> {
-  static constexpr int W = NSK_COL_W4096, T = NSK_COL_T4096, NWG = NSK_COL_NWG4096;
-};
-template <> struct ColCfg<2048> { static constexpr int W = 1, T = 256, NWG = 512; };
-template <> struct ColCfg<1024> { static constexpr int W = 2, T = 256, NWG = 256; };
-template <> struct ColCfg<512> { static constexpr int W = 2, T = 256, NWG = 128; };
-template <> struct ColCfg<256> { static constexpr int W = 16, T = 512, NWG = 8; };
-template <> struct ColCfg<128> { static constexpr int W = 32, T = 512, NWG = 2; };
-template <> struct ColCfg<64> { static constexpr int W = 32, T = 512, NWG = 1; };
-template <> struct ColCfg<32> { static constexpr int W = 16, T = 256, NWG = 1; };
-template <> struct ColCfg<16> { static constexpr int W = 8, T = 256, NWG = 1; };
-template <> struct ColCfg<8> { static constexpr int W = 4, T = 256, NWG = 1; };
+template <int N>
+struct ColGeo;
+template <> struct ColGeo<4096> { static constexpr int E = NSK_COL_E4096, W = NSK_COL_W4096, NWG = NSK_COL_NWG4096; static constexpr bool PF = NSK_COL_PF; };
+template <> struct ColGeo<2048> { static constexpr int E = 16, W = 2, NWG = 512; static constexpr bool PF = false; };
+template <> struct ColGeo<1024> { static constexpr int E = 16, W = 4, NWG = 128; static constexpr bool PF = false; };
+template <> struct ColGeo<512> { static constexpr int E = 16, W = 8, NWG = 32; static constexpr bool PF = false; };
+template <> struct ColGeo<256> { static constexpr int E = 16, W = 16, NWG = 8; static constexpr bool PF = false; };
+template <> struct ColGeo<128> { static constexpr int E = 16, W = 32, NWG = 2; static constexpr bool PF = false; };
+template <> struct ColGeo<64> { static constexpr int E = 16, W = 32, NWG = 1; static constexpr bool PF = false; };
+template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; static constexpr bool PF = false; };
+template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false; };
+template <> struct ColGeo<8> { static constexpr int E = 16, W = 4, NWG = 1; static constexpr bool PF = false; };
 
 template <int N>
-constexpr int row_bands_c() { return N / (RowCfg<N>::G * RowCfg<N>::ITER); }
+constexpr int row_bands_c() { return N / (RowGeo<N>::LG * RowGeo<N>::ROWS); }
 
 inline int row_bands(int N) {
   switch (N) {

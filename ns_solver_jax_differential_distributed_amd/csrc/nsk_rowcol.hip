@@ -2,10 +2,20 @@
 // size.  Compiled once per N with -DNSK_N=<N>; each object carries its own
 // copy of the twiddle table (g_tw) and an initialiser for it.
 //
+// Spectrum layout ("paired row spectra"): for row x, Z = FFT_y(a + i b) of
+// a pair of real fields, stored as float4 {Z(c), Z(N-c)} at c = 1..N/2-1
+// and {Z(0), Z(N/2)} at c = 0 (8*N*N bytes per pair).  The row pass moves
+// Z values only: a row is one complex line and needs no split into the two
+// fields.  The column pass reads the float4 of column c and separates the
+// half spectra X(c) = (Z(c) + conj Z(N-c))/2, Y(c) = (Z(c) - conj Z(N-c))/2i
+// of the two fields before transforming along x; at c = 0 the lines are
+// X(0) + i X(N/2) and Y(0) + i Y(N/2) (all four are real sequences).
+//
 // Data flow of one rollout step n (NS_solver.py:742-747 and 671-697):
 //
-//   row pass  (k_row, RK_STEP_ROLL): spectrum of u_n (row-spectral form)
-//       -> inverse row FFT of rows x-1..x+1 -> u_n rows in LDS
+//   row pass  (k_row, RK_STEP_ROLL): spectrum of u_n
+//       -> inverse row FFT of row x+1 into a ring of three rows held in
+//          registers (x-1, x, x+1); y-neighbours by lane shuffles
 //       -> objective partial (u_n . f_n, before the step)
 //       -> predictor u*_n = u_n + dt (nu lap u_n + adv(u_n) + f_n)
 //       -> forward row FFT -> spectrum of u*_n
@@ -16,7 +26,7 @@
 //          Laplacian of SpectralPoissonSolver, k = 0 untouched)
 //       -> inverse column FFT -> spectrum of u_{n+1}
 //
-// The state between steps never leaves the row-spectral form, so one step
+// The state between steps never leaves the spectral form, so one step
 // reads and writes 8 bytes per cell in each pass.
 #include "nsk_fft.h"
 #include "nsk_internal.h"
@@ -75,17 +85,26 @@ __device__ inline void forcing_dtheta(float A, float B, float t, float x,
   dB = f * d * d / (sig * sig * sig) * (-pi2 * (double)t * sin(2.0 * bb));
 }
 
-// Deterministic workgroup sum (fixed shuffle tree, then waves in order);
-// the result is valid in thread 0.
+// Deterministic workgroup sum (fixed shuffle tree per wave, then the waves
+// in order; one slot per thread below a full wave); the result is valid in
+// thread 0.
 template <int T>
 __device__ __forceinline__ double block_sum(double v, double* red, int tid) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if ((tid & 63) == 0) red[tid >> 6] = v;
-  __syncthreads();
   double s = 0.0;
-  if (tid == 0) {
-    for (int w = 0; w < T / 64; ++w) s += red[w];
+  if constexpr (T >= 64) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < T / 64; ++w) s += red[w];
+    }
+  } else {
+    red[tid] = v;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < T; ++w) s += red[w];
+    }
   }
   __syncthreads();
   return s;
@@ -164,132 +183,100 @@ __device__ __forceinline__ float2 predictor_vjp(float2 c, float2 l, float2 r, fl
   return make_float2(c.x + dt * (nu * lapx + ax), c.y + dt * (nu * lapy + ay));
 }
 
-// Split the forward FFT Z of (a + i b) into the half spectra of a and b and
-// store them in the packed float4 layout.
+// Float2 offset, within a row of the spectrum, of Z(y).
 template <int N>
-__device__ __forceinline__ float4 split_pair(const float2* L, int c) {
+__device__ __forceinline__ int zslot(int y) {
   constexpr int HN = N / 2;
-  if (c == 0) {
-    const float2 z0 = L[pad(0)], zn = L[pad(HN)];
-    return make_float4(z0.x, zn.x, z0.y, zn.y);
-  }
-  const float2 zk = L[pad(c)], zm = L[pad(N - c)];
-  return make_float4(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y), 0.5f * (zk.y + zm.y),
-                     0.5f * (zm.x - zk.x));
+  return y < HN ? 2 * y : 2 * ((N - y) & (HN - 1)) + 1;
 }
 
-// Inverse of split_pair: expand one packed float4 of column c into Z(c) and
-// Z(N-c) of (a + i b) (Hermitian symmetry of each real field).
-template <int N>
-__device__ __forceinline__ void join_pair(float2* L, int c, float4 h) {
-  constexpr int HN = N / 2;
-  if (c == 0) {
-    L[pad(0)] = make_float2(h.x, h.z);
-    L[pad(HN)] = make_float2(h.y, h.w);
-  } else {
-    L[pad(c)] = make_float2(h.x - h.w, h.y + h.z);
-    L[pad(N - c)] = make_float2(h.x + h.w, h.z - h.y);
-  }
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src), __shfl(v.y, src));
 }
 
-// Row-pass helpers: G rows move between global memory and the LDS ring
-// through registers, so the loads of the next rows can be issued before the
-// current rows are processed.
-template <int N, int G, int T>
-struct SpecRows {
-  static constexpr int HN = N / 2, PF = (G * HN + T - 1) / T;
-  float4 v[PF];
-  __device__ __forceinline__ void fetch(const float4* src, int rfirst, int tid) {
-#pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int e = tid + q * T;
-      if ((G * HN) % T == 0 || e < G * HN) {
-        const int l = e / HN, c = e % HN;
-        v[q] = src[(size_t)((rfirst + l) & (N - 1)) * HN + c];
-      }
-    }
+// Raw buffer view of one slot's array: a per-thread 32-bit byte offset
+// (VGPR) plus a uniform one (SGPR), so that the loads and stores of a row
+// share one address register instead of a 64-bit address per point.
+// Accesses past `bytes` read 0 / are dropped by the hardware.
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ void init(const void* p, unsigned bytes) {
+    r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
   }
-  // expand into the row FFT input Z of (a + i b) in ring slots
-  template <int RING>
-  __device__ __forceinline__ void commit(float2* ring, int sfirst, int tid) const {
-    constexpr int NP = LineLen<N>::NP;
-#pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int e = tid + q * T;
-      if ((G * HN) % T == 0 || e < G * HN) {
-        const int l = e / HN, c = e % HN;
-        join_pair<N>(ring + ((sfirst + l) % RING) * NP, c, v[q]);
-      }
-    }
+  __device__ __forceinline__ float ld1(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ float2 ld2(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ float4 ld4(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ void st1(float v, unsigned voff, unsigned soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  }
+  __device__ __forceinline__ void st2(float2 v, unsigned voff, unsigned soff) const {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, voff, soff, 0);
+  }
+  // 16 bytes as two 8-byte stores.  A 16-byte buffer store with an SGPR
+  // offset may have its data registers overwritten by the next VALU
+  // instruction without a wait state in between (the compiler treats that
+  // form as hazard-free); on gfx950 this corrupted stored lanes.  Stores of
+  // at most 8 bytes do not have the hazard.
+  __device__ __forceinline__ void st4(float4 v, unsigned voff, unsigned soff) const {
+    st2(make_float2(v.x, v.y), voff, soff);
+    st2(make_float2(v.z, v.w), voff + 8u, soff);
   }
 };
 
-template <int N, int G, int T>
-struct RealRows {
-  static constexpr int Q = N / 4, PF = (G * Q + T - 1) / T;
-  float4 vx[PF], vy[PF];
-  __device__ __forceinline__ void fetch(const float* src, bool two, int rfirst, int tid) {
-    const size_t NN = (size_t)N * N;
-#pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int e = tid + q * T;
-      if ((G * Q) % T == 0 || e < G * Q) {
-        const int l = e / Q, y4 = e % Q;
-        const size_t off = (size_t)((rfirst + l) & (N - 1)) * N + 4 * y4;
-        vx[q] = *reinterpret_cast<const float4*>(src + off);
-        vy[q] = two ? *reinterpret_cast<const float4*>(src + NN + off) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
-  }
-  template <int RING>
-  __device__ __forceinline__ void commit(float2* ring, int sfirst, int tid) const {
-    constexpr int NP = LineLen<N>::NP;
-#pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int e = tid + q * T;
-      if ((G * Q) % T == 0 || e < G * Q) {
-        const int l = e / Q, y = 4 * (e % Q);
-        float2* L = ring + ((sfirst + l) % RING) * NP;
-        L[pad(y)] = make_float2(vx[q].x, vy[q].x);
-        L[pad(y + 1)] = make_float2(vx[q].y, vy[q].y);
-        L[pad(y + 2)] = make_float2(vx[q].z, vy[q].z);
-        L[pad(y + 3)] = make_float2(vx[q].w, vy[q].w);
-      }
-    }
-  }
+// LDS line stride (float2) with stride % 16 == skew % 16: lines used by the
+// same 16 lanes then start in different banks.
+constexpr int line_stride(int np, int skew) { return np + (((skew - np) % 16) + 16) % 16; }
+
+template <int N>
+struct RowK {
+  using G = RowGeo<N>;
+  using F = RF<N, G::E>;
+  static constexpr int EE = F::EE, TL = F::T, LG = G::LG, ROWS = G::ROWS;
+  static constexpr int T = LG * TL;
+  static constexpr int NW = TL >= 64 ? TL / 64 : 0;  // waves per row stream
+  static constexpr int LS = line_stride(LineLen<N>::NP, TL < 16 ? TL : 0);
+  static constexpr int WPE = (EE == 16) ? 2 : 4;  // waves per SIMD aimed at (register budget)
 };
 
-// The row pass.  A workgroup owns a band of G*ITER consecutive rows and
-// marches through it G rows at a time.  The LDS ring holds RING = G + 2*HALO
-// rows of real pairs (u_x, u_y): rows xb-HALO .. xb+G-1+HALO for the rows
-// xb .. xb+G-1 being processed.  Outputs of the predictor go back into the
-// slots of rows xb-1 .. xb+G-2, which no later row needs; after they have
-// been transformed and stored, the next rows (prefetched into registers at
-// the top of the iteration) take the same slots.  Row r sits in slot
-// (r - x0 + HALO) % RING.
+// The row pass.  Workgroup `band` runs LG row streams; stream g walks the
+// ROWS rows x = xs .. xs+ROWS-1.  For the stepping kinds each thread keeps
+// its points (y = t + TL*m) of rows x-1, x and x+1 in registers (rp, rc,
+// rn); the y-neighbours of row x come from the adjacent lanes (shuffles),
+// and across wave boundaries from an LDS edge table written right after
+// the row was produced (two parities, so that row x+1's edges never
+// overwrite row x's before they are read; at least one barrier separates
+// every write from its reads).
 template <int N, int KIND, int SCHEME>
-__global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
-  constexpr int G = RowCfg<N>::G, ITER = RowCfg<N>::ITER, T = RowCfg<N>::T;
-  constexpr int NP = LineLen<N>::NP, HN = N / 2;
+__global__ void __launch_bounds__(RowK<N>::T)
+    __attribute__((amdgpu_waves_per_eu(RowK<N>::WPE))) k_row(RowArgs a) {
+  using K = RowK<N>;
+  constexpr int E = RowGeo<N>::E, EE = K::EE, TL = K::TL, LG = K::LG, ROWS = K::ROWS, T = K::T;
+  constexpr int NW = K::NW, LS = K::LS, HN = N / 2;
+  constexpr bool DB = RowGeo<N>::DB;
   constexpr int SRC = (KIND == RK_REAL_TO_SPEC || KIND == RK_STEP_FROM_REAL || KIND == RK_STEP_FORCE)
                           ? SRC_REAL : SRC_SPEC;
   constexpr int OP = (KIND == RK_REAL_TO_SPEC || KIND == RK_SPEC_TO_REAL) ? OP_COPY
                      : (KIND == RK_ADJ ? OP_ADJ : OP_STEP);
   constexpr bool FFIELD = (KIND == RK_STEP_FORCE);
   constexpr bool ROLL = (KIND == RK_STEP_ROLL);
-  constexpr int HALO = (OP == OP_COPY) ? 0 : 1;
-  constexpr int RING = G + 2 * HALO;
-  constexpr bool TWR = NSK_ROW_TWR && ITER > 1;  // twiddles reused across iterations
-  constexpr int GD = (OP == OP_ADJ) ? G : 1;
-  constexpr int PH = (G == 2) ? 2 : 1;      // stencil phases
-  constexpr int GP = G / PH;                // rows per phase
-  constexpr int RES = (GP * N + T - 1) / T;
-  __shared__ float2 lds[RING * NP];
-  __shared__ float s_f[G];
-  __shared__ double s_dA[GD], s_dB[GD];
-  __shared__ double s_red[T / 64];
+  static_assert(NW == 0 || LG == 1, "edge table assumes one stream per workgroup");
+  static_assert(NW == 0 || RF<N, E>::S > 1, "edge table needs the exchange barriers");
+  __shared__ float2 s_x[(DB ? 2 : 1) * LG * LS];
+  __shared__ float2 s_edge[NW > 0 ? 2 * NW * 2 * EE : 1];  // [parity][wave][first,last][m]
+  __shared__ float s_f[LG * ROWS];
+  __shared__ double s_dA[OP == OP_ADJ ? LG * ROWS : 1], s_dB[OP == OP_ADJ ? LG * ROWS : 1];
+  __shared__ double s_red[T >= 64 ? T / 64 : T];
+  __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
 
   const int tid = threadIdx.x;
+  const int t = (LG == 1) ? tid : tid % TL, g = (LG == 1) ? 0 : tid / TL, lane = tid & 63;
   const int b = blockIdx.y;
   const int nb = gridDim.x;
   const int band = xcd_remap(blockIdx.x, nb);
@@ -311,228 +298,273 @@ __global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
   }
   const nsk_plan& p = a.plan;
   const float dt = sl.dt, nu = sl.nu;
-  const float t = (a.t_explicit >= 0.0f) ? a.t_explicit : (float)n * dt;
+  const float tm = (a.t_explicit >= 0.0f) ? a.t_explicit : (float)n * dt;
   const size_t NN = (size_t)N * N;
 
   const float* srcR = (SRC == SRC_REAL) ? a.src_real + (long long)b * a.src_bstride : nullptr;
-  const float4* srcS = (SRC == SRC_SPEC) ? a.src_spec + (size_t)b * N * HN : nullptr;
-  float4* dstS = a.dst_spec ? a.dst_spec + (size_t)b * N * HN : nullptr;
+  const float2* srcZ = (SRC == SRC_SPEC) ? reinterpret_cast<const float2*>(a.src_spec) + (size_t)b * NN : nullptr;
+  float2* dstZ = a.dst_spec ? reinterpret_cast<float2*>(a.dst_spec) + (size_t)b * NN : nullptr;
   float* dstR = a.dst_real ? a.dst_real + (long long)b * a.dst_bstride : nullptr;
   const bool two = a.src_nc > 1;
+  const bool two_out = a.dst_nc > 1;
+
+  const int xs = (band * LG + g) * ROWS;  // first row of this stream
+
+  Twid<N, E> tw;
+  tw.load(t, s_tw, tid, T);
+  XBuf<DB> xb;
+  xb.b0 = s_x + g * LS;
+  xb.bstride = LG * LS;
+  xb.lstride = 0;
+  xb.k = 0;
+
+  // Z(y) of row x sits at float2 2y (y < N/2) or 2((N-y) mod N/2) + 1; for
+  // y = t + TL*m that is byte 16t + 16*TL*m (m < EE/2),
+  // 16*TL*(EE-1-m) + 16*(TL-1-t) + 24 (m > EE/2, and m = EE/2 with t > 0)
+  // and 8 for m = EE/2, t = 0 (Z(N/2) in the second half of column 0).
+  Buf bz_in, bz_out, br_in, br_out;
+  if (srcZ) bz_in.init(srcZ, (unsigned)(NN * 8));
+  if (dstZ) bz_out.init(dstZ, (unsigned)(NN * 8));
+  if (srcR) br_in.init(srcR, (unsigned)(NN * 4 * (two ? 2 : 1)));
+  if (dstR) br_out.init(dstR, (unsigned)(NN * 4 * (two_out ? 2 : 1)));
+  const unsigned zlo = 16u * t, zhi = 16u * (TL - 1 - t) + 24u;
+  const unsigned zmid = (t == 0) ? 8u : zhi + 16u * TL * (EE / 2 - 1);
+  auto zoff = [&](int m, unsigned row) -> unsigned {
+    return m < EE / 2 ? row + zlo : (m == EE / 2 ? row + zmid : row + zhi);
+  };
+  auto zsoff = [&](int m) -> unsigned {
+    return m < EE / 2 ? 16u * TL * m : (m == EE / 2 ? 0u : 16u * TL * (EE - 1 - m));
+  };
+  auto load_spec = [&](float2 (&v)[EE], int x) {
+    const unsigned row = (unsigned)(x & (N - 1)) * (8u * N);
+#pragma unroll
+    for (int m = 0; m < EE; ++m) v[m] = bz_in.ld2(zoff(m, row), zsoff(m));
+  };
+  auto store_spec = [&](const float2 (&v)[EE], int x) {
+    const unsigned row = (unsigned)x * (8u * N);
+#pragma unroll
+    for (int m = 0; m < EE; ++m) bz_out.st2(v[m], zoff(m, row), zsoff(m));
+  };
+  auto load_real = [&](float2 (&v)[EE], int x) {
+    const unsigned off = (unsigned)(x & (N - 1)) * (4u * N) + 4u * t;
+#pragma unroll
+    for (int m = 0; m < EE; ++m)
+      v[m] = make_float2(br_in.ld1(off, 4u * TL * m), two ? br_in.ld1(off, 4u * TL * m + 4u * (unsigned)NN) : 0.0f);
+  };
+  auto store_real = [&](const float2 (&v)[EE], int x) {
+    const unsigned off = (unsigned)x * (4u * N) + 4u * t;
+#pragma unroll
+    for (int m = 0; m < EE; ++m) {
+      br_out.st1(v[m].x, off, 4u * TL * m);
+      if (two_out) br_out.st1(v[m].y, off, 4u * TL * m + 4u * (unsigned)NN);
+    }
+  };
+
+  // ---- copies (one row at a time, no neighbours)
+  if (OP == OP_COPY || (ROLL && final_mode)) {
+    __syncthreads();  // twiddle table
+#pragma unroll 1
+    for (int i = 0; i < ROWS; ++i) {
+      const int x = xs + i;
+      float2 v[1][EE];
+      if constexpr (SRC == SRC_SPEC) {
+        load_spec(v[0], x);
+        rfft<N, E, +1, 1, DB>(v, xb, t, tw);
+        store_real(v[0], x);
+      } else {
+        load_real(v[0], x);
+        rfft<N, E, -1, 1, DB>(v, xb, t, tw);
+        store_spec(v[0], x);
+      }
+    }
+    return;
+  }
+
+  // ---- per-row forcing (f_x depends on x only, f_y = 0)
+  if (tid < LG * ROWS) {
+    const int x = band * LG * ROWS + tid;
+    const float xc = p.Lf * ((float)x / (float)N);
+    s_f[tid] = FFIELD ? 0.0f : forcing_x(sl.A, sl.B, tm, xc, p);
+    if constexpr (OP == OP_ADJ) {
+      double dA, dB;
+      forcing_dtheta(sl.A, sl.B, tm, xc, p, dA, dB);
+      s_dA[tid] = dA;
+      s_dB[tid] = dB;
+    }
+  }
+  __syncthreads();
 
   float* st = nullptr;
-  if (OP == OP_STEP && a.ustore != nullptr && !final_mode) {
+  if (OP == OP_STEP && a.ustore != nullptr) {
     const int rel = n - a.st_base;
     if (rel >= 0 && rel % a.st_every == 0 && rel / a.st_every < a.st_cap)
       st = a.ustore + (long long)b * a.st_bstride + (long long)(rel / a.st_every) * 2 * NN;
   }
-  const float* us = nullptr;
-  if (OP == OP_ADJ) us = a.ustate + (long long)b * a.us_bstride + (long long)(n - a.us_base) * 2 * NN;
-
-  const int x0 = band * G * ITER;
-  float2* ring = lds;
-  double acc0 = 0.0, acc1 = 0.0;
+  Buf bst;
+  if (st) bst.init(st, (unsigned)(NN * 8));
+  Buf bus;
+  if (OP == OP_ADJ) bus.init(a.ustate + (long long)b * a.us_bstride + (long long)(n - a.us_base) * 2 * NN,
+                             (unsigned)(NN * 8));
+  const float* Fb = FFIELD ? a.force + (long long)b * a.force_bstride : nullptr;
+  const double seed = (double)a.jseed * (double)sl.jseed;
+  const float scale_ke = (a.objective == NSK_OBJ_MEAN_KE)
+                             ? (float)(seed / ((double)sl.nsteps * (double)NN)) : 0.0f;
+  const float jdir = (float)(seed * (double)dt * p.dx * p.dx);
   const double dx2 = p.dx * p.dx;
 
-  TwCache<N, G, T> tc;
-  if constexpr (TWR) tc.load(tid);
-
-  // ---- prologue: rows x0-HALO .. x0+G-1+HALO into slots 0 .. RING-1
-  if constexpr (SRC == SRC_SPEC) {
-    for (int e = tid; e < RING * HN; e += T) {
-      const int l = e / HN, c = e % HN;
-      join_pair<N>(ring + l * NP, c, srcS[(size_t)((x0 - HALO + l) & (N - 1)) * HN + c]);
-    }
-    __syncthreads();
-    if constexpr (HALO == 0 || G == 2) {
-#pragma unroll
-      for (int part = 0; part < RING / G; ++part) {
-        if constexpr (TWR)
-          fft_lines_tc<N, +1, G, T, RING>(ring, part * G, tid, tc);
-        else
-          fft_lines<N, +1, G, T, RING>(ring, part * G, tid);
-      }
+  auto fetch_row = [&](float2 (&v)[EE], int x) {
+    NSK_SCHED_FENCE();
+    if constexpr (SRC == SRC_SPEC) {
+      load_spec(v, x);
+      rfft<N, E, +1, 1, DB>(&v, xb, t, tw);
     } else {
-      fft_lines<N, +1, RING, T, RING>(ring, 0, tid);
+      load_real(v, x);
     }
-  } else {
-    const float* s1 = srcR + NN;
-    for (int e = tid; e < RING * (N / 4); e += T) {
-      const int l = e / (N / 4), y = 4 * (e % (N / 4));
-      const size_t off = (size_t)((x0 - HALO + l) & (N - 1)) * N + y;
-      const float4 vx = *reinterpret_cast<const float4*>(srcR + off);
-      const float4 vy = two ? *reinterpret_cast<const float4*>(s1 + off) : make_float4(0.f, 0.f, 0.f, 0.f);
-      float2* L = ring + l * NP;
-      L[pad(y)] = make_float2(vx.x, vy.x);
-      L[pad(y + 1)] = make_float2(vx.y, vy.y);
-      L[pad(y + 2)] = make_float2(vx.z, vy.z);
-      L[pad(y + 3)] = make_float2(vx.w, vy.w);
+    NSK_SCHED_FENCE();
+  };
+  auto publish = [&](const float2 (&v)[EE], int x) {
+    if constexpr (NW > 0) {
+      const int w = t >> 6, par = x & 1;
+      if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < EE; ++m) s_edge[((par * NW + w) * 2 + 0) * EE + m] = v[m];
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int m = 0; m < EE; ++m) s_edge[((par * NW + w) * 2 + 1) * EE + m] = v[m];
+      }
     }
-    __syncthreads();
-  }
+  };
+  // Row x (in c) at y-1 and y+1 of this thread's point m (y = t + TL*m).
+  // Lanes of one row stream are a contiguous lane range; the lane holding
+  // y-1 is the previous one, except at the stream's first thread (its y-1
+  // is the last thread's point m-1).  Wave boundaries inside a stream go
+  // through the edge table.
+  const int lb = (TL < 64) ? (lane & ~(TL - 1)) : 0;
+  const int sd = (TL < 64) ? lb + ((t - 1) & (TL - 1)) : ((lane - 1) & 63);
+  const int su = (TL < 64) ? lb + ((t + 1) & (TL - 1)) : ((lane + 1) & 63);
+  auto nbr = [&](const float2 (&c)[EE], int m, int x, float2& d, float2& u) {
+    if constexpr (TL == 1) {
+      d = c[(m + EE - 1) % EE];
+      u = c[(m + 1) % EE];
+    } else if constexpr (NW == 0) {
+      const float2 d0 = shfl2(c[m], sd), d1 = shfl2(c[(m + EE - 1) % EE], sd);
+      const float2 u0 = shfl2(c[m], su), u1 = shfl2(c[(m + 1) % EE], su);
+      d = (t != 0) ? d0 : d1;
+      u = (t != TL - 1) ? u0 : u1;
+    } else {
+      const int w = t >> 6, par = x & 1;
+      d = shfl2(c[m], sd);
+      u = shfl2(c[m], su);
+      // lane 0: thread t-1 (wave w-1, lane 63), or for t = 0 the last
+      // thread's point m-1; lane 63: thread t+1 (wave w+1, lane 0), or for
+      // t = TL-1 thread 0's point m+1
+      const float2 de = (w > 0) ? s_edge[((par * NW + w - 1) * 2 + 1) * EE + m]
+                                : s_edge[((par * NW + NW - 1) * 2 + 1) * EE + (m + EE - 1) % EE];
+      const float2 ue = (w < NW - 1) ? s_edge[((par * NW + w + 1) * 2 + 0) * EE + m]
+                                     : s_edge[((par * NW) * 2 + 0) * EE + (m + 1) % EE];
+      if (lane == 0) d = de;
+      if (lane == 63) u = ue;
+    }
+  };
 
+  double acc0 = 0.0, acc1 = 0.0;
+  float2 rp[EE], rc[EE], rn[EE];
+  fetch_row(rp, xs - 1);
+  fetch_row(rc, xs);
+  publish(rc, xs);
 #pragma unroll 1
-  for (int it = 0; it < ITER; ++it) {
-    const int xb = x0 + it * G;
-    const bool has_next = it + 1 < ITER;
-    const int s0 = (xb - x0) % RING;           // first slot freed by this iteration
-    const int rnext = xb + G + HALO;           // first row loaded for the next one
-
-    // ---- prefetch the next rows into registers
-    SpecRows<N, G, T> pfs;
-    RealRows<N, G, T> pfr;
-    if (has_next) {
-      if constexpr (SRC == SRC_SPEC)
-        pfs.fetch(srcS, rnext, tid);
-      else
-        pfr.fetch(srcR, two, rnext, tid);
-    }
-
-    if constexpr (OP == OP_COPY) {
-      if constexpr (SRC == SRC_SPEC) {
-        const bool two_out = a.dst_nc > 1;
-        for (int e = tid; e < G * N; e += T) {
-          const int g = e / N, y = e % N;
-          const float2 v = ring[((s0 + g) % RING) * NP + pad(y)];
-          dstR[(size_t)(xb + g) * N + y] = v.x;
-          if (two_out) dstR[NN + (size_t)(xb + g) * N + y] = v.y;
-        }
-      } else {
-        if constexpr (TWR)
-          fft_lines_tc<N, -1, G, T, RING>(ring, s0, tid, tc);
-        else
-          fft_lines<N, -1, G, T, RING>(ring, s0, tid);
-        for (int e = tid; e < G * HN; e += T) {
-          const int g = e / HN, c = e % HN;
-          dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(ring + ((s0 + g) % RING) * NP, c);
+  for (int i = 0; i < ROWS; ++i) {
+    const int x = xs + i;
+    fetch_row(rn, x + 1);
+    publish(rn, x + 1);
+    if constexpr (SRC == SRC_REAL && NW > 0) __syncthreads();
+    const int r = g * ROWS + i;
+    if constexpr (OP == OP_STEP) {
+      // checkpoint of u_n and the objective's term before the step (one
+      // uniform branch each, outside the per-point loop)
+      if (st) {
+        const unsigned st_row = (unsigned)x * (4u * N) + 4u * t;
+#pragma unroll
+        for (int m = 0; m < EE; ++m) {
+          bst.st1(rc[m].x, st_row, 4u * TL * m);
+          bst.st1(rc[m].y, st_row, 4u * TL * m + 4u * (unsigned)NN);
         }
       }
-    } else if (ROLL && final_mode) {
-      for (int e = tid; e < G * N; e += T) {
-        const int g = e / N, y = e % N;
-        const float2 v = ring[((s0 + 1 + g) % RING) * NP + pad(y)];
-        dstR[(size_t)(xb + g) * N + y] = v.x;
-        dstR[NN + (size_t)(xb + g) * N + y] = v.y;
+      const float fx0 = s_f[r];
+      if constexpr (!FFIELD) {
+        if (a.objective == NSK_OBJ_WORK) {
+          double sx = 0.0;
+#pragma unroll
+          for (int m = 0; m < EE; ++m) sx += (double)rc[m].x;
+          acc0 += (double)fx0 * sx;
+        } else {
+#pragma unroll
+          for (int m = 0; m < EE; ++m)
+            acc0 += 0.5 * ((double)rc[m].x * (double)rc[m].x + (double)rc[m].y * (double)rc[m].y);
+        }
+      }
+      NSK_SCHED_FENCE();
+#pragma unroll
+      for (int m = 0; m < EE; ++m) {
+        const int y = t + TL * m;
+        float2 dn, up;
+        nbr(rc, m, x, dn, up);
+        float fx = fx0, fy = 0.0f;
+        if constexpr (FFIELD) {
+          fx = Fb[(size_t)x * N + y];
+          fy = Fb[NN + (size_t)x * N + y];
+        }
+        rp[m] = predictor<SCHEME>(rc[m], rp[m], rn[m], dn, up, fx, fy, nu, dt, p);
+        if ((m & 3) == 3) NSK_SCHED_FENCE();
       }
     } else {
-      // ---- per-row forcing (f_x depends on x only, f_y = 0)
-      if (tid < G) {
-        const int x = xb + tid;
-        const float xc = p.Lf * ((float)x / (float)N);
-        s_f[tid] = FFIELD ? 0.0f : forcing_x(sl.A, sl.B, t, xc, p);
-        if constexpr (OP == OP_ADJ) {
-          double dA, dB;
-          forcing_dtheta(sl.A, sl.B, t, xc, p, dA, dB);
-          s_dA[tid] = dA;
-          s_dB[tid] = dB;
-        }
-      }
-      __syncthreads();
-
-      // ---- stencil, PH phases of GP rows: outputs are held in registers
-      // until the phase's reads are done, then written into the slots of
-      // rows xb-1+g, which later rows no longer read.
-      const float* F = FFIELD ? a.force + (long long)b * a.force_bstride : nullptr;
-      const double seed = (double)a.jseed * (double)sl.jseed;
-      const float scale_ke = (a.objective == NSK_OBJ_MEAN_KE)
-                                 ? (float)(seed / ((double)sl.nsteps * (double)NN)) : 0.0f;
-      const float jdir = (float)(seed * (double)dt * dx2);
+      // adjoint: forward state U at the five stencil points (buffer loads
+      // with 32-bit offsets); the objective's direct terms by uniform
+      // coefficients (no per-point branch)
+      const unsigned ox = (unsigned)x * (4u * N), oxm = (unsigned)((x - 1) & (N - 1)) * (4u * N);
+      const unsigned oxp = (unsigned)((x + 1) & (N - 1)) * (4u * N);
+      const unsigned c1 = 4u * (unsigned)NN;
+      const double dA = s_dA[r], dB = s_dB[r];
+      const bool work = (a.objective == NSK_OBJ_WORK);
+      const float wx = work ? jdir * s_f[r] : 0.0f;
+      const float ke = work ? 0.0f : scale_ke;
+      const double gw = work ? seed * (double)dt * dx2 : 0.0;
 #pragma unroll
-      for (int ph = 0; ph < PH; ++ph) {
-        float2 res[RES];
-#pragma unroll
-        for (int q = 0; q < RES; ++q) {
-          const int e = tid + q * T;
-          if ((GP * N) % T == 0 || e < GP * N) {
-            const int g = ph * GP + e / N, y = e % N;
-            const int x = xb + g;
-            const float2* Lm = ring + ((s0 + g) % RING) * NP;      // row x-1
-            const float2* Lc = ring + ((s0 + g + 1) % RING) * NP;  // row x
-            const float2* Lp = ring + ((s0 + g + 2) % RING) * NP;  // row x+1
-            const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
-            const float2 c = Lc[pad(y)], l = Lm[pad(y)], r = Lp[pad(y)];
-            const float2 d = Lc[pad(ym)], u = Lc[pad(yp)];
-            if constexpr (OP == OP_STEP) {
-              float fx = s_f[g], fy = 0.0f;
-              if constexpr (FFIELD) {
-                fx = F[(size_t)x * N + y];
-                fy = F[NN + (size_t)x * N + y];
-              } else {
-                if (a.objective == NSK_OBJ_WORK)
-                  acc0 += (double)fx * (double)c.x;
-                else
-                  acc0 += 0.5 * ((double)c.x * (double)c.x + (double)c.y * (double)c.y);
-              }
-              if (st) {
-                st[(size_t)x * N + y] = c.x;
-                st[NN + (size_t)x * N + y] = c.y;
-              }
-              res[q] = predictor<SCHEME>(c, l, r, d, u, fx, fy, nu, dt, p);
-            } else {
-              const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
-              const float* U0 = us;
-              const float* U1 = us + NN;
-              const float2 Uc = make_float2(U0[(size_t)x * N + y], U1[(size_t)x * N + y]);
-              const float2 Ul = make_float2(U0[(size_t)xm * N + y], U1[(size_t)xm * N + y]);
-              const float2 Ur = make_float2(U0[(size_t)xp * N + y], U1[(size_t)xp * N + y]);
-              const float2 Ud = make_float2(U0[(size_t)x * N + ym], U1[(size_t)x * N + ym]);
-              const float2 Uu = make_float2(U0[(size_t)x * N + yp], U1[(size_t)x * N + yp]);
-              float2 v = predictor_vjp<SCHEME>(c, l, r, d, u, Uc, Ul, Ur, Ud, Uu, nu, dt, p);
-              double grow;
-              if (a.objective == NSK_OBJ_WORK) {
-                v.x += jdir * s_f[g];
-                grow = seed * (double)dt * dx2 * (double)Uc.x + (double)dt * (double)c.x;
-              } else {
-                v.x += scale_ke * Uc.x;
-                v.y += scale_ke * Uc.y;
-                grow = (double)dt * (double)c.x;
-              }
-              acc0 += grow * s_dA[g];
-              acc1 += grow * s_dB[g];
-              res[q] = v;
-            }
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < RES; ++q) {
-          const int e = tid + q * T;
-          if ((GP * N) % T == 0 || e < GP * N) {
-            const int g = ph * GP + e / N, y = e % N;
-            ring[((s0 + g) % RING) * NP + pad(y)] = res[q];
-          }
-        }
-        __syncthreads();
-      }
-      if constexpr (TWR)
-        fft_lines_tc<N, -1, G, T, RING>(ring, s0, tid, tc);
-      else
-        fft_lines<N, -1, G, T, RING>(ring, s0, tid);
-      for (int e = tid; e < G * HN; e += T) {
-        const int g = e / HN, c = e % HN;
-        dstS[(size_t)(xb + g) * HN + c] = split_pair<N>(ring + ((s0 + g) % RING) * NP, c);
+      for (int m = 0; m < EE; ++m) {
+        const unsigned y = (unsigned)(t + TL * m);
+        const unsigned oy = 4u * y, oym = 4u * ((y - 1u) & (N - 1)), oyp = 4u * ((y + 1u) & (N - 1));
+        const float2 Uc = make_float2(bus.ld1(ox + oy, 0), bus.ld1(ox + oy, c1));
+        const float2 Ul = make_float2(bus.ld1(oxm + oy, 0), bus.ld1(oxm + oy, c1));
+        const float2 Ur = make_float2(bus.ld1(oxp + oy, 0), bus.ld1(oxp + oy, c1));
+        const float2 Ud = make_float2(bus.ld1(ox + oym, 0), bus.ld1(ox + oym, c1));
+        const float2 Uu = make_float2(bus.ld1(ox + oyp, 0), bus.ld1(ox + oyp, c1));
+        const float2 c = rc[m];
+        float2 dn, up;
+        nbr(rc, m, x, dn, up);
+        float2 v = predictor_vjp<SCHEME>(c, rp[m], rn[m], dn, up, Uc, Ul, Ur, Ud, Uu, nu, dt, p);
+        v.x += wx + ke * Uc.x;
+        v.y += ke * Uc.y;
+        const double grow = gw * (double)Uc.x + (double)dt * (double)c.x;
+        acc0 += grow * dA;
+        acc1 += grow * dB;
+        rp[m] = v;
+        if ((m & 3) == 3) NSK_SCHED_FENCE();
       }
     }
-    __syncthreads();
-
-    // ---- next rows into the freed slots
-    if (has_next) {
-      if constexpr (SRC == SRC_SPEC)
-        pfs.template commit<RING>(ring, s0, tid);
-      else
-        pfr.template commit<RING>(ring, s0, tid);
-      __syncthreads();
-      if constexpr (SRC == SRC_SPEC) {
-        if constexpr (TWR)
-          fft_lines_tc<N, +1, G, T, RING>(ring, s0, tid, tc);
-        else
-          fft_lines<N, +1, G, T, RING>(ring, s0, tid);
-      }
+    NSK_SCHED_FENCE();
+    rfft<N, E, -1, 1, DB>(&rp, xb, t, tw);
+    store_spec(rp, x);
+    NSK_SCHED_FENCE();
+#pragma unroll
+    for (int m = 0; m < EE; ++m) {
+      rp[m] = rc[m];
+      rc[m] = rn[m];
     }
   }
 
   if constexpr (OP == OP_STEP && !FFIELD) {
-    if (a.part != nullptr && !final_mode) {
+    if (a.part != nullptr) {
       const double s = block_sum<T>(acc0, s_red, tid);
       if (tid == 0 && n < a.pcap) {
         const double scale = (a.objective == NSK_OBJ_WORK)
@@ -555,19 +587,12 @@ __global__ void __launch_bounds__(RowCfg<N>::T) k_row(RowArgs a) {
   }
 }
 
-// Spectral multiplier at mode (kx, ky) applied to the pair (X, Y).
-template <int N, int MODE>
-__device__ __forceinline__ void apply_mode(float2& X, float2& Y, int kx, int ky,
-                                           const nsk_plan& p) {
-  constexpr int HN = N / 2;
-  const int kxs = kx < HN ? kx : kx - N;
-  const int kys = ky < HN ? ky : ky - N;
-  const bool zero = (kxs == 0 && kys == 0);
-  const float k2 = (float)(kxs * kxs + kys * kys) * p.kscale;
+// Spectral multiplier at one mode: sx, sy = sin(k dx)/dx per direction,
+// k2 = |k|^2 of the spectral Laplacian, zero = (k == 0).
+template <int MODE>
+__device__ __forceinline__ void apply_mode(float2& X, float2& Y, float sx, float sy, float k2, bool zero) {
   if constexpr (MODE == COL_PROJECT) {
     if (zero) return;
-    const float sx = -g_tw[kx * (TW_N / N)].y * p.inv_dx;  // sin(2 pi kx / N) / dx
-    const float sy = -g_tw[ky * (TW_N / N)].y * p.inv_dx;
     const float inv = 1.0f / k2;
     const float2 dot = make_float2(sx * X.x + sy * Y.x, sx * X.y + sy * Y.y);
     const float ax = sx * inv, ay = sy * inv;
@@ -580,21 +605,30 @@ __device__ __forceinline__ void apply_mode(float2& X, float2& Y, int kx, int ky,
   }
 }
 
-// The column pass.  Each workgroup handles blocks of W spectral columns
-// (both fields: 2W lines of N points), blocks NWG apart, and prefetches the
-// next block into registers while it transforms the current one.
-#if NSK_COL_WPE > 0
-#define NSK_COL_ATTR __attribute__((amdgpu_waves_per_eu(NSK_COL_WPE)))
-#else
-#define NSK_COL_ATTR
-#endif
+template <int N>
+struct ColK {
+  using G = ColGeo<N>;
+  using F = RF<N, G::E>;
+  static constexpr int EE = F::EE, TL = F::T, W = G::W;
+  static constexpr int T = W * TL;
+  static constexpr int LS = line_stride(LineLen<N>::NP, W <= 16 ? 16 / W : 1);
+};
+
+// The column pass.  Workgroup w handles blocks of W spectral columns c
+// (lines X and Y of each), blocks NWG apart; thread (kw = tid % W, t)
+// holds points x = t + TL*m of both lines of column c = kb*W + kw, so the
+// loads and stores of one row segment are W*16 contiguous bytes.  The next
+// block is loaded into registers while the current one is transformed.
 template <int N, int MODE>
-__global__ void __launch_bounds__(ColCfg<N>::T) NSK_COL_ATTR k_col(ColArgs a) {
-  constexpr int W = ColCfg<N>::W, T = ColCfg<N>::T, NL = 2 * W;
-  constexpr int NP = LineLen<N>::NP, HN = N / 2, NBLK = HN / W;
-  constexpr int PF = (N * W + T - 1) / T;
-  __shared__ float2 lds[NL * NP];
-  const int tid = threadIdx.x, b = blockIdx.y;
+__global__ void __launch_bounds__(ColK<N>::T) k_col(ColArgs a) {
+  using K = ColK<N>;
+  constexpr int E = ColGeo<N>::E, EE = K::EE, TL = K::TL, W = K::W, LS = K::LS;
+  constexpr int HN = N / 2, NBLK = HN / W;
+  constexpr bool PERSIST = ColGeo<N>::NWG < NBLK;
+  constexpr bool PF = PERSIST && ColGeo<N>::PF;
+  __shared__ float2 s_x[2 * W * LS];
+  __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
+  const int tid = threadIdx.x, kw = tid % W, t = tid / W, b = blockIdx.y;
   const int nwg = gridDim.x;
   const int w0 = xcd_remap(blockIdx.x, nwg);
   if (a.require == REQ_ACTIVE) {
@@ -602,87 +636,115 @@ __global__ void __launch_bounds__(ColCfg<N>::T) NSK_COL_ATTR k_col(ColArgs a) {
     const int n = sl.step + a.step_off;
     if (n < 0 || n >= sl.nsteps) return;
   }
+  if (w0 >= NBLK) return;
   float4* H = a.spec + (size_t)b * N * HN;
   const nsk_plan& p = a.plan;
   const float scale = 1.0f / ((float)N * (float)N);
 
-  float4 pf[PF];
+  Twid<N, E> tw;
+  tw.load(t, s_tw, tid, K::T);
+  __syncthreads();
+  XBuf<false> xb;
+  xb.b0 = s_x + kw * LS;
+  xb.bstride = 0;
+  xb.lstride = W * LS;
+  xb.k = 0;
+  // (cos, -sin)(2 pi t / N): sin(2 pi kx / N) for kx = t + TL*m follows by
+  // rotating by 2 pi m / EE
+  const float2 ct = g_tw[t * (TW_N / N)];
+
+  Buf bh;
+  bh.init(H, (unsigned)((size_t)N * HN * 16));
+  const unsigned hrow = 16u * (unsigned)(t * HN + kw);  // + 16*(kb*W) per block
+  float4 pf[EE];
   auto fetch = [&](int kb) {
+    const unsigned off = hrow + 16u * (unsigned)(kb * W);
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
-      const int e = tid + q * T;
-      if ((N * W) % T == 0 || e < N * W) {
-        const int x = e / W, kw = e % W;
-        pf[q] = H[(size_t)x * HN + kb * W + kw];
-      }
-    }
+    for (int m = 0; m < EE; ++m) pf[m] = bh.ld4(off, 16u * TL * HN * m);
   };
-  // one block per workgroup: load straight into LDS; otherwise stage the
-  // next block in registers while the current one is transformed
-  constexpr bool PERSIST = ColCfg<N>::NWG < NBLK;
-  if (PERSIST && w0 < NBLK) fetch(w0);
+  if (PF) fetch(w0);
 #pragma unroll 1
-  for (int kb = w0; kb < NBLK; kb += nwg) {
-    if constexpr (PERSIST) {
+  for (int kb = w0; kb < NBLK; kb = PERSIST ? kb + nwg : NBLK) {
+    const int c = kb * W + kw;
+    if (!PF) fetch(kb);
+    float2 v[2][EE];
+    // the half spectra of the two fields; the factor 1/2 of the separation
+    // is folded into the final scale (exact: a power of two)
+    if (c == 0) {
 #pragma unroll
-      for (int q = 0; q < PF; ++q) {
-        const int e = tid + q * T;
-        if ((N * W) % T == 0 || e < N * W) {
-          const int x = e / W, kw = e % W;
-          lds[(2 * kw) * NP + pad(x)] = make_float2(pf[q].x, pf[q].y);
-          lds[(2 * kw + 1) * NP + pad(x)] = make_float2(pf[q].z, pf[q].w);
-        }
+      for (int m = 0; m < EE; ++m) {
+        v[0][m] = make_float2(pf[m].x, pf[m].z);
+        v[1][m] = make_float2(pf[m].y, pf[m].w);
       }
     } else {
-      for (int e = tid; e < N * W; e += T) {
-        const int x = e / W, kw = e % W;
-        const float4 h = H[(size_t)x * HN + kb * W + kw];
-        lds[(2 * kw) * NP + pad(x)] = make_float2(h.x, h.y);
-        lds[(2 * kw + 1) * NP + pad(x)] = make_float2(h.z, h.w);
+#pragma unroll
+      for (int m = 0; m < EE; ++m) {
+        v[0][m] = make_float2(pf[m].x + pf[m].z, pf[m].y - pf[m].w);
+        v[1][m] = make_float2(pf[m].y + pf[m].w, pf[m].z - pf[m].x);
       }
     }
-    __syncthreads();
-    if (PERSIST && kb + nwg < NBLK) fetch(kb + nwg);
-    fft_lines<N, -1, NL, T, NL>(lds, 0, tid);
-    for (int e = tid; e < W * N; e += T) {
-      const int kw = e / N, kx = e % N;
-      const int ky = kb * W + kw;
-      float2* LX = lds + (2 * kw) * NP;
-      float2* LY = LX + NP;
-      if (ky == 0) {
-        // column 0 carries ky = 0 (real part of the row data) and ky = N/2
-        // (imaginary part); separate them, treat each, recombine.
-        if (kx > HN) continue;
-        const int kx2 = (N - kx) & (N - 1);
-        const float2 x1 = LX[pad(kx)], x2 = LX[pad(kx2)];
-        const float2 y1 = LY[pad(kx)], y2 = LY[pad(kx2)];
-        float2 AX = make_float2(0.5f * (x1.x + x2.x), 0.5f * (x1.y - x2.y));
-        float2 BX = make_float2(0.5f * (x1.y + x2.y), 0.5f * (x2.x - x1.x));
-        float2 AY = make_float2(0.5f * (y1.x + y2.x), 0.5f * (y1.y - y2.y));
-        float2 BY = make_float2(0.5f * (y1.y + y2.y), 0.5f * (y2.x - y1.x));
-        apply_mode<N, MODE>(AX, AY, kx, 0, p);
-        apply_mode<N, MODE>(BX, BY, kx, HN, p);
-        LX[pad(kx)] = make_float2((AX.x - BX.y) * scale, (AX.y + BX.x) * scale);
-        LY[pad(kx)] = make_float2((AY.x - BY.y) * scale, (AY.y + BY.x) * scale);
-        if (kx2 != kx) {
-          LX[pad(kx2)] = make_float2((AX.x + BX.y) * scale, (BX.x - AX.y) * scale);
-          LY[pad(kx2)] = make_float2((AY.x + BY.y) * scale, (BY.x - AY.y) * scale);
+    if (PF && kb + nwg < NBLK) fetch(kb + nwg);
+    rfft<N, E, -1, 2, false>(v, xb, t, tw);
+
+    const float sc = (c == 0) ? scale : 0.5f * scale;
+    const int kys = c;  // c < N/2
+    const float sy = -g_tw[c * (TW_N / N)].y * p.inv_dx;
+    if (kb == 0) {
+      // column 0 carries ky = 0 (real part of each line) and ky = N/2
+      // (imaginary part): with F(kx) the column transform of A + iB,
+      // A(kx) = (F(kx) + conj F(N-kx))/2 and B(kx) = (F(kx) - conj F(N-kx))/2i.
+      float2* B = xb.next();
+      if (kw == 0) {
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+#pragma unroll
+          for (int m = 0; m < EE; ++m) B[l * xb.lstride + pad(t + TL * m)] = v[l][m];
+      }
+      __syncthreads();
+      if (kw == 0) {
+#pragma unroll
+        for (int m = 0; m < EE; ++m) {
+          const int kx = t + TL * m;
+          const int kp = (N - kx) & (N - 1);
+          const int kxs = kx < HN ? kx : kx - N;
+          const float sx = (-ct.y * w16c(m * (16 / EE)) + ct.x * w16s(m * (16 / EE))) * p.inv_dx;
+          const float2 x1 = v[0][m], x2 = B[pad(kp)];
+          const float2 y1 = v[1][m], y2 = B[xb.lstride + pad(kp)];
+          float2 AX = make_float2(0.5f * (x1.x + x2.x), 0.5f * (x1.y - x2.y));
+          float2 BX = make_float2(0.5f * (x1.y + x2.y), 0.5f * (x2.x - x1.x));
+          float2 AY = make_float2(0.5f * (y1.x + y2.x), 0.5f * (y1.y - y2.y));
+          float2 BY = make_float2(0.5f * (y1.y + y2.y), 0.5f * (y2.x - y1.x));
+          apply_mode<MODE>(AX, AY, sx, 0.0f, (float)(kxs * kxs) * p.kscale, kx == 0);
+          apply_mode<MODE>(BX, BY, sx, 0.0f, (float)(kxs * kxs + HN * HN) * p.kscale, false);
+          v[0][m] = make_float2(AX.x - BX.y, AX.y + BX.x);
+          v[1][m] = make_float2(AY.x - BY.y, AY.y + BY.x);
+          if ((m & 3) == 3) NSK_SCHED_FENCE();
         }
-      } else {
-        float2 X = LX[pad(kx)], Y = LY[pad(kx)];
-        apply_mode<N, MODE>(X, Y, kx, ky, p);
-        LX[pad(kx)] = make_float2(X.x * scale, X.y * scale);
-        LY[pad(kx)] = make_float2(Y.x * scale, Y.y * scale);
       }
     }
-    __syncthreads();
-    fft_lines<N, +1, NL, T, NL>(lds, 0, tid);
-    for (int e = tid; e < N * W; e += T) {
-      const int x = e / W, kw = e % W;
-      const float2 X = lds[(2 * kw) * NP + pad(x)], Y = lds[(2 * kw + 1) * NP + pad(x)];
-      H[(size_t)x * HN + kb * W + kw] = make_float4(X.x, X.y, Y.x, Y.y);
+    if (c != 0) {
+#pragma unroll
+      for (int m = 0; m < EE; ++m) {
+        const int kx = t + TL * m;
+        const int kxs = kx < HN ? kx : kx - N;
+        const float sx = (-ct.y * w16c(m * (16 / EE)) + ct.x * w16s(m * (16 / EE))) * p.inv_dx;
+        apply_mode<MODE>(v[0][m], v[1][m], sx, sy, (float)(kxs * kxs + kys * kys) * p.kscale, false);
+        if ((m & 3) == 3) NSK_SCHED_FENCE();
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int m = 0; m < EE; ++m) v[l][m] = make_float2(v[l][m].x * sc, v[l][m].y * sc);
+    rfft<N, E, +1, 2, false>(v, xb, t, tw);
+    const unsigned off = hrow + 16u * (unsigned)(kb * W);
+#pragma unroll
+    for (int m = 0; m < EE; ++m) {
+      const float2 X = v[0][m], Y = v[1][m];
+      bh.st4((c == 0) ? make_float4(X.x, Y.x, X.y, Y.y)
+                      : make_float4(X.x - Y.y, X.y + Y.x, X.x + Y.y, Y.x - X.y),
+             off, 16u * TL * HN * m);
+    }
   }
 }
 
@@ -696,8 +758,8 @@ constexpr int NN_ = NSK_N;
 namespace {
 template <int KIND, int SCHEME>
 hipError_t go_row(const RowArgs& a, int batch, hipStream_t s) {
-  const int bands = NN_ / (RowCfg<NN_>::G * RowCfg<NN_>::ITER);
-  hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(RowCfg<NN_>::T), 0, s, a);
+  const int bands = row_bands_c<NN_>();
+  hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(RowK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 
@@ -724,7 +786,7 @@ hipError_t NSK_CAT(launch_row_, NSK_N)(int kind, int scheme, const RowArgs& a, i
 namespace {
 template <int MODE>
 hipError_t go_col(const ColArgs& a, int batch, hipStream_t s) {
-  hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColCfg<NN_>::NWG, batch), dim3(ColCfg<NN_>::T), 0, s, a);
+  hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 """Host-side logic that runs without a GPU: the reference's config and error
 contracts (unit_tests/test_distributed_solver.py:462-562), the C ABI library
 (loads, exports every symbol of include/nsk.h, host-only plan setup), and a
-NumPy model of the row-spectral layout the kernels use."""
+NumPy model of the spectrum layout the kernels use."""
 import ctypes
 import json
 import math
@@ -151,20 +151,18 @@ def test_slot_record_layout():
     assert rec[0, 4].item() == 5 and rec[0, 5].item() == 100
 
 
-# ---------------------------------------------------------------- row-spectral layout model
+# ---------------------------------------------------------------- spectrum layout model
 def _row_fwd(u):
-    """Host model of the row pass output: per row, the half spectra of u_x and
-    u_y packed as float4, with ky=0 holding the (real) DC and Nyquist terms."""
+    """Host model of the row pass output (include/nsk.h): per row x,
+    Z = FFT_y(u_x + i u_y) stored as float4 {Z(c), Z(N-c)} at c >= 1 and
+    {Z(0), Z(N/2)} at c = 0."""
     N = u.shape[-1]
     HN = N // 2
     Z = np.fft.fft(u[0] + 1j * u[1], axis=1)
     H = np.zeros((N, HN, 4))
-    H[:, 0] = np.stack([Z[:, 0].real, Z[:, HN].real, Z[:, 0].imag, Z[:, HN].imag], -1)
-    for c in range(1, HN):
-        zk, zm = Z[:, c], Z[:, N - c]
-        X = (zk + np.conj(zm)) / 2
-        Y = (zk - np.conj(zm)) / 2j
-        H[:, c] = np.stack([X.real, X.imag, Y.real, Y.imag], -1)
+    for c in range(HN):
+        zc, zm = Z[:, c], Z[:, HN if c == 0 else N - c]
+        H[:, c] = np.stack([zc.real, zc.imag, zm.real, zm.imag], -1)
     return H
 
 
@@ -172,17 +170,17 @@ def _row_inv(H):
     N = H.shape[0]
     HN = N // 2
     Z = np.zeros((N, N), complex)
-    Z[:, 0] = H[:, 0, 0] + 1j * H[:, 0, 2]
-    Z[:, HN] = H[:, 0, 1] + 1j * H[:, 0, 3]
-    for c in range(1, HN):
-        h = H[:, c]
-        Z[:, c] = (h[:, 0] - h[:, 3]) + 1j * (h[:, 1] + h[:, 2])
-        Z[:, N - c] = (h[:, 0] + h[:, 3]) + 1j * (h[:, 2] - h[:, 1])
+    for c in range(HN):
+        Z[:, c] = H[:, c, 0] + 1j * H[:, c, 1]
+        Z[:, HN if c == 0 else N - c] = H[:, c, 2] + 1j * H[:, c, 3]
     z = np.fft.ifft(Z, axis=1)
     return np.stack([z.real, z.imag])
 
 
 def _col(H, L, mode):
+    """Host model of the column pass: separate the half spectra X, Y of the
+    two fields (at c = 0 the lines X(0) + i X(N/2) and Y(0) + i Y(N/2)),
+    transform along x, apply the multiplier, transform back, recombine."""
     N = H.shape[0]
     HN = N // 2
     dx = L / N
@@ -204,8 +202,13 @@ def _col(H, L, mode):
 
     out = np.zeros_like(H)
     for c in range(HN):
-        CX = np.fft.fft(H[:, c, 0] + 1j * H[:, c, 1])
-        CY = np.fft.fft(H[:, c, 2] + 1j * H[:, c, 3])
+        zc = H[:, c, 0] + 1j * H[:, c, 1]
+        zm = H[:, c, 2] + 1j * H[:, c, 3]
+        if c == 0:
+            lx, ly = zc.real + 1j * zm.real, zc.imag + 1j * zm.imag
+        else:
+            lx, ly = (zc + np.conj(zm)) / 2, (zc - np.conj(zm)) / 2j
+        CX, CY = np.fft.fft(lx), np.fft.fft(ly)
         NX = np.zeros(N, complex)
         NY = np.zeros(N, complex)
         if c == 0:
@@ -220,16 +223,20 @@ def _col(H, L, mode):
         else:
             for kx in range(N):
                 NX[kx], NY[kx] = app(CX[kx], CY[kx], kx, c)
-        NX, NY = np.fft.ifft(NX), np.fft.ifft(NY)
-        out[:, c] = np.stack([NX.real, NX.imag, NY.real, NY.imag], -1)
+        X, Y = np.fft.ifft(NX), np.fft.ifft(NY)
+        if c == 0:
+            zc, zm = X.real + 1j * Y.real, X.imag + 1j * Y.imag
+        else:
+            zc, zm = X + 1j * Y, np.conj(X) + 1j * np.conj(Y)
+        out[:, c] = np.stack([zc.real, zc.imag, zm.real, zm.imag], -1)
     return out
 
 
 @pytest.mark.parametrize("N", [8, 16])
 def test_layout_model_round_trip_projection_poisson(N):
-    """The packed row-spectral form (include/nsk.h) with the column pass's
-    DC/Nyquist separation reproduces the oracle's projection and Poisson
-    solve exactly (FP64)."""
+    """The paired row-spectral form (include/nsk.h) with the column pass's
+    field separation and DC/Nyquist handling reproduces the oracle's
+    projection and Poisson solve exactly (FP64)."""
     L = 2 * math.pi
     rng = np.random.default_rng(N)
     u = rng.standard_normal((2, N, N))
