@@ -35,6 +35,13 @@
 #error "compile with -DNSK_N=<grid size>"
 #endif
 
+#ifndef NSK_ST128
+#define NSK_ST128 1
+#endif
+#ifndef NSK_COL_RCP
+#define NSK_COL_RCP 1
+#endif
+
 #define NSK_CAT2(a, b) a##b
 #define NSK_CAT(a, b) NSK_CAT2(a, b)
 
@@ -225,8 +232,15 @@ struct Buf {
   // form as hazard-free); on gfx950 this corrupted stored lanes.  Stores of
   // at most 8 bytes do not have the hazard.
   __device__ __forceinline__ void st4(float4 v, unsigned voff, unsigned soff) const {
+#if NSK_ST128
+    // one 16-byte store with the whole offset in the VGPR and a constant 0
+    // SGPR offset, the form the compiler guards with a wait state
+    using V4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), r, voff + soff, 0, 0);
+#else
     st2(make_float2(v.x, v.y), voff, soff);
     st2(make_float2(v.z, v.w), voff + 8u, soff);
+#endif
   }
 };
 
@@ -593,7 +607,13 @@ template <int MODE>
 __device__ __forceinline__ void apply_mode(float2& X, float2& Y, float sx, float sy, float k2, bool zero) {
   if constexpr (MODE == COL_PROJECT) {
     if (zero) return;
+#if NSK_COL_RCP
+    // reciprocal estimate refined by one Newton step
+    float inv = __builtin_amdgcn_rcpf(k2);
+    inv = inv * (2.0f - k2 * inv);
+#else
     const float inv = 1.0f / k2;
+#endif
     const float2 dot = make_float2(sx * X.x + sy * Y.x, sx * X.y + sy * Y.y);
     const float ax = sx * inv, ay = sy * inv;
     X = make_float2(X.x - ax * dot.x, X.y - ax * dot.y);
@@ -611,7 +631,11 @@ struct ColK {
   using F = RF<N, G::E>;
   static constexpr int EE = F::EE, TL = F::T, W = G::W;
   static constexpr int T = W * TL;
+#ifdef NSK_COL_SKEW
+  static constexpr int LS = line_stride(LineLen<N>::NP, NSK_COL_SKEW);
+#else
   static constexpr int LS = line_stride(LineLen<N>::NP, W <= 16 ? 16 / W : 1);
+#endif
 };
 
 // The column pass.  Workgroup w handles blocks of W spectral columns c

@@ -423,8 +423,9 @@ def poisson_requests(n: int, rate: float, seed: int = 0, grad_frac: float = 0.3,
     """The served workload: Poisson arrivals at ``rate`` per second; N drawn
     with ``weights``; steps uniform in steps_range; seeded u0 (Taylor-Green
     + perturbation, max|u0| <= 1.2), theta ~ U[0.5,1.5]^3, nu log-uniform
-    [1e-4, 1e-2], dt at CFL 0.3 capped by the diffusive limit and by the
-    stability limit of central advection, 0.9 * 2 nu / 1.2^2."""
+    [1e-4, 1e-2], dt from workloads.stable_dt: CFL <= 0.3, the diffusive limit
+    and the stability limit of central advection, all for the speed bound
+    1.2 + steps*dt that the unit-amplitude forcing allows over the run."""
     from . import workloads
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -436,7 +437,7 @@ def poisson_requests(n: int, rate: float, seed: int = 0, grad_frac: float = 0.3,
         th = rng.uniform(0.5, 1.5, 3).astype(np.float32)
         nu = float(np.exp(rng.uniform(math.log(1e-4), math.log(1e-2))))
         dx = L / N
-        dt = float(np.float32(min(0.3 * dx / 1.2, dx * dx / (4 * nu), 0.9 * 2 * nu / 1.44)))
+        dt = float(np.float32(workloads.stable_dt(dx, 1.2, nu, 0.3, steps)))
         out.append(SolveRequest(N=N, steps=steps, theta=th, nu=nu, dt=dt, seed=seed * 100003 + i,
                                 want_grad=bool(rng.random() < grad_frac), arrival=t))
     return out

@@ -62,16 +62,45 @@ def nu_batch(n: int, seed: int = 0, lo: float = 1e-4, hi: float = 1e-2) -> np.nd
     return np.exp(r).astype(np.float32)
 
 
+def stable_dt(dx: float, umax0: float, nu: float | None, cfl: float = 0.3, steps: int = 0,
+              C: float = 1.0, stable: bool = True) -> float:
+    """Largest dt with cfl*dx/u >= dt, dx^2/(4 nu) >= dt and (stable) 0.9 * 2 nu / u^2 >= dt,
+    where u = umax0 + C*steps*dt bounds the speed over the whole run: the Gaussian forcing
+    (amplitude C, NS_solver.py:621-640) can add at most C per unit time.  The last condition
+    is the von Neumann limit of explicit Euler with central advection ((u dt/dx)^2 <= 2 nu
+    dt/dx^2); without it a long central rollout at small nu grows until it overflows."""
+    def ok(dt):
+        u = umax0 + C * steps * dt
+        if dt > cfl * dx / u:
+            return False
+        if nu is not None:
+            if dt > dx * dx / (4.0 * nu):
+                return False
+            if stable and dt > 0.9 * 2.0 * nu / (u * u):
+                return False
+        return True
+    hi = cfl * dx / umax0
+    if ok(hi):
+        return hi
+    lo = 0.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if ok(mid) else (lo, mid)
+    return lo
+
+
 def cfl_dt(u0: np.ndarray, L: float, cfl: float = 0.3, nu: float | None = None,
-           stable: bool = False) -> float:
+           stable: bool = False, steps: int = 0, C: float = 1.0) -> float:
     """dt = cfl*dx/max|u0|, also <= dx^2/(4 nu) when nu is given.  With
-    stable=True also dt <= 0.9 * 2 nu / max|u0|^2: explicit Euler with central
-    advection amplifies every mode unless the diffusion keeps up (von Neumann:
-    (u dt/dx)^2 <= 2 nu dt/dx^2), and trajectories whose results are compared
-    against each other must stay finite."""
+    stable=True also dt <= 0.9 * 2 nu / max|u|^2 (explicit Euler with central
+    advection amplifies every mode unless the diffusion keeps up).  With
+    steps > 0 the speed bound is max|u0| + C*steps*dt (forcing amplitude C),
+    so the conditions hold for the whole rollout (stable_dt)."""
     N = u0.shape[-1]
     dx = L / N
     umax = float(np.abs(u0).max())
+    if steps > 0:
+        return float(np.float32(stable_dt(dx, umax, nu, cfl, steps, C, stable)))
     dt = cfl * dx / umax
     if nu is not None:
         dt = min(dt, dx * dx / (4.0 * nu))

@@ -1,0 +1,81 @@
+"""Code-generation checks on the gfx950 assembly of the N=4096 kernels
+(CPU only: hipcc cross-compiles).
+
+* A 12- or 16-byte VMEM store whose data registers are overwritten by the
+  very next (VALU) instruction: with an SGPR offset the compiler treats
+  that sequence as hazard-free, and on gfx950 it corrupted stored lanes.
+* The hot kernels (fused row pass, column projection) must not spill.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "ns_solver_jax_differential_distributed_amd", "csrc", "nsk_rowcol.hip")
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+@pytest.fixture(scope="module")
+def asm_4096(tmp_path_factory):
+    hipcc = _hipcc()
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("asm") / "rc4096.s"
+    cmd = [hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-DNSK_N=4096",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "--cuda-device-only", "-S", SRC, "-o", str(out)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    return out.read_text()
+
+
+def _instructions(text):
+    for line in text.split("\n"):
+        s = line.split(";")[0].strip()
+        if s and not s.endswith(":") and not s.startswith("."):
+            yield s
+
+
+def _vregs(op):
+    m = re.match(r"v\[(\d+):(\d+)\]", op)
+    if m:
+        return set(range(int(m.group(1)), int(m.group(2)) + 1))
+    m = re.match(r"v(\d+)$", op)
+    return {int(m.group(1))} if m else set()
+
+
+def test_no_wide_store_data_overwritten_by_next_instruction(asm_4096):
+    ins = list(_instructions(asm_4096))
+    found = []
+    for cur, nxt in zip(ins, ins[1:]):
+        m = re.match(r"(buffer|global|flat)_store_dwordx([34])\s+(.*)", cur)
+        if not m:
+            continue
+        ops = [o.strip() for o in m.group(3).split(",")]
+        data = _vregs(ops[0] if m.group(1) == "buffer" else ops[1])
+        if nxt.startswith("v_"):
+            dst = _vregs(nxt.split(None, 1)[1].split(",")[0].strip())
+            if data & dst:
+                found.append((cur, nxt))
+    assert not found, found[:5]
+
+
+@pytest.mark.parametrize("kernel", [
+    "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi4ELi0EEEvNS_7RowArgsE",  # fused row pass, central
+    "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi4ELi1EEEvNS_7RowArgsE",  # fused row pass, upwind
+    "_ZN3nsk12_GLOBAL__N_15k_colILi4096ELi0EEEvNS_7ColArgsE",      # column projection
+])
+def test_hot_kernels_do_not_spill(asm_4096, kernel):
+    m = re.search(r"\.amdhsa_kernel " + re.escape(kernel) + r"\n(.*?)\.end_amdhsa_kernel", asm_4096, re.S)
+    assert m, kernel
+    priv = re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", m.group(1))
+    assert priv and int(priv.group(1)) == 0, f"{kernel} uses scratch ({priv.group(1) if priv else '?'} bytes)"
