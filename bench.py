@@ -5,8 +5,9 @@ MI355X at N=4096, float32.
 Workload (the "plain method on GPU, large grids" configuration at its
 largest size): a 4096 x 4096 periodic field on [0, 2 pi)^2, Taylor-Green
 plus a seeded divergence-free perturbation (modes |k| <= 8, amplitude 0.1),
-central advection, nu = 1e-4, dt from CFL 0.3, Gaussian forcing with theta
-drawn from U[0.5, 1.5].  One step = one full Chorin step (predictor,
+central advection, nu = 1e-4, dt for CFL <= 0.3 and explicit-Euler
+stability over the whole run, Gaussian forcing with theta drawn from
+U[0.5, 1.5].  One step = one full Chorin step (predictor,
 pressure projection, objective accumulation) of that field; the timed
 region is one rollout of --steps steps with every input already in HBM.
 
@@ -16,6 +17,11 @@ Also reported (one JSON line):
                 stream; traffic from profiles/traffic_N4096.json if present
   cpu_baseline  the FP32 NumPy/SciPy oracle timed on this host for a bounded
                 number of steps of the same workload
+  accuracy      the GPU result against the FP64 oracle over a few steps of
+                the same workload (relative L2 of u, J error, divergence)
+  batched, gradient, serving
+                the batched-rollout, rollout+adjoint and continuous-batching
+                configurations (see the leg_* docstrings)
 """
 from __future__ import annotations
 
@@ -42,6 +48,8 @@ def parse():
     ap.add_argument("--scheme", default="central", choices=["central", "upwind"])
     ap.add_argument("--nu", type=float, default=1e-4)
     ap.add_argument("--reps", type=int, default=50, help="launches per pass for the roofline timing")
+    ap.add_argument("--accuracy-steps", type=int, default=8,
+                    help="steps of the FP64-oracle comparison run beside the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -92,6 +100,32 @@ def cpu_baseline(u0, theta, N, L, nu, dt, scheme, budget_s):
     return {"value": N * N * steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": f"FP32 NumPy/SciPy oracle rollout, N={N}, {steps} steps "
                       f"({el:.1f} s), scipy.fft workers={cores}"}
+
+
+def accuracy(solver, u0, u0_dev, theta, N, L, nu, dt, scheme, steps):
+    """The GPU float32 rollout against the FP64 oracle from the same u0 over
+    `steps` steps at the bench size: relative L2 of the final velocity,
+    relative error of J, and the divergence of the final field (central
+    differences, in units of max|u|/dx)."""
+    import numpy as np
+    import torch
+    from oracle import ns_oracle as O
+    J, uf = solver.rollout(u0_dev, theta, steps=steps)
+    torch.cuda.synchronize()
+    ug = uf.double().cpu().numpy()
+    cores = len(os.sched_getaffinity(0))
+    O.set_fft_workers(cores)
+    try:
+        t0 = time.perf_counter()
+        J64, u64 = O.rollout(u0.astype(np.float64), theta, N, L, O.PhysParams(nu), float(dt), steps, scheme)[:2]
+        el = time.perf_counter() - t0
+    finally:
+        O.set_fft_workers(None)
+    div = (np.roll(ug[0], -1, 0) - np.roll(ug[0], 1, 0) + np.roll(ug[1], -1, 1) - np.roll(ug[1], 1, 1)) / 2.0
+    return {"steps": steps, "rel_l2_u_vs_fp64": float(np.linalg.norm(ug - u64) / np.linalg.norm(u64)),
+            "rel_err_J_vs_fp64": float(abs(float(J) - J64) / abs(J64)),
+            "max_div_dx_over_umax": float(np.abs(div).max() / np.abs(ug).max()),
+            "fp64_oracle_s": round(el, 1)}
 
 
 def pass_times(solver, reps):
@@ -230,7 +264,7 @@ def leg_serving(n=300, rate=200.0):
             if eng is None:
                 eng = engines[r.N] = E.RolloutEngine(r.N, L, 1)
             if r.want_grad:
-                out.append(eng.rollout_and_grad(r.u0, r.nu, r.dt, r.theta, r.steps, 25))
+                out.append(eng.rollout_and_grad(r.u0, r.nu, r.dt, r.theta, r.steps, "auto"))
             else:
                 out.append(eng.rollout(r.u0, r.nu, r.dt, r.theta, r.steps))
         return out
@@ -312,6 +346,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(u0, theta, N, L, nu, dt, args.scheme, args.cpu_seconds)
+        line["accuracy"] = accuracy(solver, u0, u0_dev, theta, N, L, nu, dt, args.scheme, args.accuracy_steps)
     if not args.no_extras:
         for key, fn in (("batched", leg_batched), ("gradient", leg_gradient), ("serving", leg_serving)):
             try:

@@ -159,7 +159,9 @@ int nsk_rollout(const nsk_plan *p, const float *u0, float *u_final,
  * slot: J_out[b], dtheta_out[2b..2b+1] = dJ/d(A,B) (double, device),
  * du0_out (nullable) = dJ/du0 as (2,N,N) float.  spec0..spec3: four
  * spectrum buffers.  ckpt: ceil(steps/ckpt_every) fields of (2,N,N) per
- * slot; seg: ckpt_every fields of (2,N,N) per slot; gpart:
+ * slot; seg: ckpt_every fields of (2,N,N) per slot (unused, may be NULL, when
+ * ckpt_every == 1: the reverse sweep then reads every u_n from ckpt and
+ * recomputes nothing); gpart:
  * 2*nsk_partials_per_slot(p, steps) doubles per slot.  slots[b].step must
  * be 0 on entry and is `steps` on return. */
 int nsk_rollout_grad(const nsk_plan *p, const float *u0, float *u_final,

@@ -491,18 +491,30 @@ class ChorinProjector:
 # ==========================
 # 6. Single-device solver (NS_solver.py:707-756)
 # ==========================
+def _check_ckpt(ckpt_every):
+    """ckpt_every: "auto" or an int >= 1."""
+    if ckpt_every == "auto":
+        return "auto"
+    if int(ckpt_every) < 1:
+        raise ValueError("ckpt_every must be >= 1 or 'auto'")
+    return int(ckpt_every)
+
+
 class SingleCoreNSSolver:
     """Single-device NS solver: rollout, objective and gradient on the GPU
     (NS_solver.py:707-756).
 
     Extra keyword arguments (not in the reference):
       objective: "work" (reference J, NS_solver.py:746) or "mean_ke"
-      ckpt_every: checkpoint interval of the adjoint's recomputation
+      ckpt_every: checkpoint interval of the adjoint's recomputation, or
+        "auto" (default): every state kept when the trajectory fits 8 GiB of
+        HBM (no recomputation, as the reference's reverse mode keeps every
+        scan state), else about sqrt(steps)
       use_graph: replay the step sequence through HIP graphs
     """
 
     def __init__(self, grid: Grid, phys: Phys, time_cfg: Time, advection_scheme: str = "central",
-                 objective: str = "work", ckpt_every: int = 50, use_graph: bool = True):
+                 objective: str = "work", ckpt_every="auto", use_graph: bool = True):
         self.grid = grid
         self.time = time_cfg
         self.phys = phys
@@ -511,14 +523,12 @@ class SingleCoreNSSolver:
         advection = make_advection_scheme(advection_scheme)
         if objective not in _lib.OBJECTIVE:
             raise ValueError(f"objective must be 'work' or 'mean_ke', got {objective!r}")
-        if int(ckpt_every) < 1:
-            raise ValueError("ckpt_every must be >= 1")
+        self.ckpt_every = _check_ckpt(ckpt_every)
         self.forcing = GaussianForcingField(grid, phys)
         self.projector = ChorinProjector(ops, poisson, advection, phys.nu, time_cfg.dt)
         self.projector._phys = (phys.mu0, phys.sigma0, phys.C)
         self.scheme = advection_scheme
         self.objective_kind = objective
-        self.ckpt_every = int(ckpt_every)
         self.use_graph = bool(use_graph)
         self._engine = None
         self.jit_objective = self.objective
@@ -600,12 +610,12 @@ class BatchedNSSolver:
     identical to B separate SingleCoreNSSolver runs."""
 
     def __init__(self, N: int, L: float, batch: int, phys: Optional[Phys] = None,
-                 advection_scheme: str = "central", objective: str = "work", ckpt_every: int = 50,
+                 advection_scheme: str = "central", objective: str = "work", ckpt_every="auto",
                  use_graph: bool = True):
         from .engine import RolloutEngine
         phys = phys or Phys(nu=0.01, mu0=0.5, sigma0=0.1, C=1.0)
         self.N, self.L, self.batch, self.phys = N, L, batch, phys
-        self.ckpt_every = int(ckpt_every)
+        self.ckpt_every = _check_ckpt(ckpt_every)
         self.engine = RolloutEngine(N, L, batch, phys.mu0, phys.sigma0, phys.C, advection_scheme,
                                     objective, use_graph)
 

@@ -666,7 +666,29 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
   }
   void* lam = spec2;
   void* lam_next = spec3;
-  for (int sg = ncheck - 1; sg >= 0; --sg) {
+  if (ckpt_every == 1) {
+    // every state is checkpointed: the reverse sweep reads u_n straight from
+    // the checkpoints, nothing is recomputed
+    HIP_TRY(set_step(slots, p->batch, 0, st));
+    for (int n = steps - 1; n >= 0; --n) {
+      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, n, REQ_ACTIVE), p->batch, st));
+      RowArgs a = row_args(p, slots);
+      a.src_spec = (const float4*)lam;
+      a.dst_spec = (float4*)lam_next;
+      a.step_off = n;
+      a.require = REQ_ACTIVE;
+      a.part = gpart;
+      a.pcap = steps;
+      a.objective = objective;
+      a.jseed = jseed;
+      a.ustate = ckpt;
+      a.us_bstride = (long long)ncheck * NN2;
+      a.us_base = 0;
+      HIP_TRY(launch_row(p->N, RK_ADJ, scheme, a, p->batch, st));
+      std::swap(lam, lam_next);
+    }
+  }
+  for (int sg = (ckpt_every == 1) ? -1 : ncheck - 1; sg >= 0; --sg) {
     const int s0 = sg * ckpt_every;
     const int s1 = (s0 + ckpt_every < steps) ? s0 + ckpt_every : steps;
     // recompute u_n, n in [s0, s1), into seg
@@ -744,7 +766,7 @@ int nsk_rollout_adjoint(const nsk_plan* p, nsk_slot* slots, int steps, int schem
   if (rc) return rc;
   if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_adjoint needs steps >= 1");
   if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
-  if (!slots || !spec0 || !spec1 || !spec2 || !spec3 || !gpart || !ckpt || !seg)
+  if (!slots || !spec0 || !spec1 || !spec2 || !spec3 || !gpart || !ckpt || (!seg && ckpt_every > 1))
     return fail(NSK_E_INVAL, "nsk_rollout_adjoint: NULL buffer");
   return adjoint_impl(p, slots, steps, scheme, objective, jseed, lam_final, dtheta_out, du0_out, spec0, spec1,
                       spec2, spec3, gpart, ckpt, seg, ckpt_every, (hipStream_t)stream);
@@ -760,7 +782,7 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
   if (steps < 1) return fail(NSK_E_INVAL, "nsk_rollout_grad needs steps >= 1");
   if (ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
   if (!u0 || !u_final || !slots || !spec0 || !spec1 || !spec2 || !spec3 || !jpart || !gpart || !ckpt ||
-      !seg || !dtheta_out)
+      (!seg && ckpt_every > 1) || !dtheta_out)
     return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
   hipStream_t st = (hipStream_t)stream;
   rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,

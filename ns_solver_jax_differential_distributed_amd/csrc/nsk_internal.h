@@ -110,16 +110,19 @@ template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, ROWS = 1; st
 #endif
 template <int N>
 struct ColGeo;
-template <> struct ColGeo<4096> { static constexpr int E = NSK_COL_E4096, W = NSK_COL_W4096, NWG = NSK_COL_NWG4096; static constexpr bool PF = NSK_COL_PF; };
-template <> struct ColGeo<2048> { static constexpr int E = 16, W = 2, NWG = 512; static constexpr bool PF = false; };
-template <> struct ColGeo<1024> { static constexpr int E = 16, W = 4, NWG = 128; static constexpr bool PF = false; };
-template <> struct ColGeo<512> { static constexpr int E = 16, W = 8, NWG = 32; static constexpr bool PF = false; };
-template <> struct ColGeo<256> { static constexpr int E = 16, W = 16, NWG = 8; static constexpr bool PF = false; };
-template <> struct ColGeo<128> { static constexpr int E = 16, W = 32, NWG = 2; static constexpr bool PF = false; };
-template <> struct ColGeo<64> { static constexpr int E = 16, W = 32, NWG = 1; static constexpr bool PF = false; };
-template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; static constexpr bool PF = false; };
-template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false; };
-template <> struct ColGeo<8> { static constexpr int E = 16, W = 4, NWG = 1; static constexpr bool PF = false; };
+#ifndef NSK_COL_XY4096
+#define NSK_COL_XY4096 0
+#endif
+template <> struct ColGeo<4096> { static constexpr int E = NSK_COL_E4096, W = NSK_COL_W4096, NWG = NSK_COL_NWG4096; static constexpr bool PF = NSK_COL_PF, XY = NSK_COL_XY4096; };
+template <> struct ColGeo<2048> { static constexpr int E = 16, W = 2, NWG = 512; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<1024> { static constexpr int E = 16, W = 4, NWG = 128; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<512> { static constexpr int E = 16, W = 8, NWG = 32; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<256> { static constexpr int E = 16, W = 16, NWG = 8; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<128> { static constexpr int E = 16, W = 32, NWG = 2; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<64> { static constexpr int E = 16, W = 32, NWG = 1; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<8> { static constexpr int E = 16, W = 4, NWG = 1; static constexpr bool PF = false, XY = false; };
 
 template <int N>
 constexpr int row_bands_c() { return N / (RowGeo<N>::LG * RowGeo<N>::ROWS); }

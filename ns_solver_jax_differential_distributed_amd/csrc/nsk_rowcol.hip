@@ -772,6 +772,174 @@ __global__ void __launch_bounds__(ColK<N>::T) k_col(ColArgs a) {
   }
 }
 
+// Adjacent-lane exchange (lanes 2i <-> 2i+1) of a float2 via DPP.
+__device__ __forceinline__ float2 swap_pair(float2 v) {
+  const int x = __builtin_amdgcn_mov_dpp(__float_as_int(v.x), 0xB1, 0xF, 0xF, false);
+  const int y = __builtin_amdgcn_mov_dpp(__float_as_int(v.y), 0xB1, 0xF, 0xF, false);
+  return make_float2(__int_as_float(x), __int_as_float(y));
+}
+
+template <int N>
+struct ColXY {
+  using G = ColGeo<N>;
+  using F = RF<N, G::E>;
+  static constexpr int EE = F::EE, TL = F::T, W = G::W;
+  static constexpr int T = 2 * W * TL;
+  // lines in one 16-lane group: 2W lines x (8/W) values of t; their LDS
+  // starts are skewed by the banks those t values occupy
+  static constexpr int LS = line_stride(LineLen<N>::NP, W <= 8 ? 8 / W : 1);
+};
+
+// The column pass with the two fields of a column in adjacent lanes: thread
+// (t, kw, xy) = (tid / 2W, (tid / 2) % W, tid % 2) holds points x = t + TL*m
+// of line xy (0: X, 1: Y) of column c = kb*W + kw.  Each lane loads and
+// stores its 8-byte half of the column's float4 ({Z(c), Z(N-c)}); the
+// projection and the recombination take the other field from the adjacent
+// lane (DPP), so only the transforms go through LDS.  One thread holds one
+// line, which halves the registers per thread and doubles the waves per
+// block against k_col.
+template <int N, int MODE>
+__global__ void __launch_bounds__(ColXY<N>::T) k_colxy(ColArgs a) {
+  using K = ColXY<N>;
+  constexpr int E = ColGeo<N>::E, EE = K::EE, TL = K::TL, W = K::W, LS = K::LS;
+  constexpr int HN = N / 2, NBLK = HN / W;
+  __shared__ float2 s_x[2 * W * LS];
+  __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
+  const int tid = threadIdx.x;
+  const int xy = tid & 1, kw = (tid >> 1) % W, t = tid / (2 * W);
+  const int b = blockIdx.y;
+  const int nwg = gridDim.x;
+  const int w0 = xcd_remap(blockIdx.x, nwg);
+  if (a.require == REQ_ACTIVE) {
+    const nsk_slot sl = a.slots[b];
+    const int n = sl.step + a.step_off;
+    if (n < 0 || n >= sl.nsteps) return;
+  }
+  float4* H = a.spec + (size_t)b * N * HN;
+  const nsk_plan& p = a.plan;
+  const float scale = 1.0f / ((float)N * (float)N);
+
+  Twid<N, E> tw;
+  tw.load(t, s_tw, tid, K::T);
+  __syncthreads();
+  XBuf<false> xb;
+  xb.b0 = s_x + (kw * 2 + xy) * LS;
+  xb.bstride = 0;
+  xb.lstride = 0;
+  xb.k = 0;
+  const float2 ct = g_tw[t * (TW_N / N)];
+  Buf bh;
+  bh.init(H, (unsigned)((size_t)N * HN * 16));
+  const unsigned hrow = 16u * (unsigned)(t * HN + kw) + 8u * xy;
+  constexpr bool PERSIST = ColGeo<N>::NWG < NBLK;
+
+#pragma unroll 1
+  for (int kb = w0; kb < NBLK; kb = PERSIST ? kb + nwg : NBLK) {
+    const int c = kb * W + kw;
+    const unsigned off = hrow + 16u * (unsigned)(kb * W);
+    float2 v[1][EE];
+    // this lane's half of {Z(c), Z(N-c)}, the other half from the partner
+#pragma unroll
+    for (int m = 0; m < EE; ++m) v[0][m] = bh.ld2(off, 16u * TL * HN * m);
+    // X(c) ~ Z(c) + conj Z(N-c), Y(c) ~ (Z(c) - conj Z(N-c))/i (the 1/2 is
+    // folded into the scale); column 0: X(0) + i X(N/2), Y(0) + i Y(N/2).
+    // Per-lane selects, no branches (c == 0 differs between the lanes).
+    const bool c0 = (c == 0);
+#pragma unroll
+    for (int m = 0; m < EE; ++m) {
+      const float2 o = swap_pair(v[0][m]);
+      const float2 zc = xy ? o : v[0][m], zm = xy ? v[0][m] : o;
+      const float gx = xy ? zc.y + zm.y : zc.x + zm.x;
+      const float gy = xy ? zm.x - zc.x : zc.y - zm.y;
+      const float ex = xy ? zc.y : zc.x, ey = xy ? zm.y : zm.x;
+      v[0][m] = make_float2(c0 ? ex : gx, c0 ? ey : gy);
+    }
+    rfft<N, E, -1, 1, false>(v, xb, t, tw);
+
+    const float sc = (c == 0) ? scale : 0.5f * scale;
+    const float sy = -g_tw[c * (TW_N / N)].y * p.inv_dx;
+    if (kb == 0) {
+      // column 0: separate ky = 0 (A) and ky = N/2 (B) with F(N-kx); at both
+      // sy = sin(0) = sin(pi) = 0, so the projection changes X only
+      float2* B = xb.next();
+      if (kw == 0) {
+#pragma unroll
+        for (int m = 0; m < EE; ++m) B[pad(t + TL * m)] = v[0][m];
+      }
+      __syncthreads();
+      if (kw == 0) {
+#pragma unroll
+        for (int m = 0; m < EE; ++m) {
+          const int kx = t + TL * m;
+          const int kp = (N - kx) & (N - 1);
+          const int kxs = kx < HN ? kx : kx - N;
+          const float2 f1 = v[0][m], f2 = B[pad(kp)];
+          float2 A = make_float2(0.5f * (f1.x + f2.x), 0.5f * (f1.y - f2.y));
+          float2 Bv = make_float2(0.5f * (f1.y + f2.y), 0.5f * (f2.x - f1.x));
+          const float k2a = (float)(kxs * kxs) * p.kscale, k2b = (float)(kxs * kxs + HN * HN) * p.kscale;
+          if constexpr (MODE == COL_PROJECT) {
+            if (xy == 0) {
+              const float sx = (-ct.y * w16c(m * (16 / EE)) + ct.x * w16s(m * (16 / EE))) * p.inv_dx;
+              if (kx != 0) {
+                const float ia = sx / k2a;
+                A = make_float2(A.x - ia * (sx * A.x), A.y - ia * (sx * A.y));
+              }
+              const float ib = sx / k2b;
+              Bv = make_float2(Bv.x - ib * (sx * Bv.x), Bv.y - ib * (sx * Bv.y));
+            }
+          } else if constexpr (MODE == COL_POISSON) {
+            const float ma = (kx == 0) ? 0.0f : -1.0f / k2a, mb = -1.0f / k2b;
+            A = make_float2(A.x * ma, A.y * ma);
+            Bv = make_float2(Bv.x * mb, Bv.y * mb);
+          }
+          v[0][m] = make_float2(A.x - Bv.y, A.y + Bv.x);
+          if ((m & 3) == 3) NSK_SCHED_FENCE();
+        }
+      }
+    }
+    if (c != 0) {
+#pragma unroll
+      for (int m = 0; m < EE; ++m) {
+        const int kx = t + TL * m;
+        const int kxs = kx < HN ? kx : kx - N;
+        const float k2 = (float)(kxs * kxs + c * c) * p.kscale;
+        if constexpr (MODE == COL_PROJECT) {
+          const float sx = (-ct.y * w16c(m * (16 / EE)) + ct.x * w16s(m * (16 / EE))) * p.inv_dx;
+          const float2 o = swap_pair(v[0][m]);
+          const float2 X = xy ? o : v[0][m], Y = xy ? v[0][m] : o;
+          const float2 dot = make_float2(sx * X.x + sy * Y.x, sx * X.y + sy * Y.y);
+#if NSK_COL_RCP
+          float inv = __builtin_amdgcn_rcpf(k2);
+          inv = inv * (2.0f - k2 * inv);
+#else
+          const float inv = 1.0f / k2;
+#endif
+          const float as = (xy ? sy : sx) * inv;
+          v[0][m] = make_float2(v[0][m].x - as * dot.x, v[0][m].y - as * dot.y);
+        } else if constexpr (MODE == COL_POISSON) {
+          const float mm = -1.0f / k2;
+          v[0][m] = make_float2(v[0][m].x * mm, v[0][m].y * mm);
+        }
+        if ((m & 3) == 3) NSK_SCHED_FENCE();
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < EE; ++m) v[0][m] = make_float2(v[0][m].x * sc, v[0][m].y * sc);
+    rfft<N, E, +1, 1, false>(v, xb, t, tw);
+    // recombine: the X lane stores Z(c) (or Z(0)), the Y lane Z(N-c) (or Z(N/2))
+#pragma unroll
+    for (int m = 0; m < EE; ++m) {
+      const float2 o = swap_pair(v[0][m]);
+      const float2 X = xy ? o : v[0][m], Y = xy ? v[0][m] : o;
+      const float gx = xy ? X.x + Y.y : X.x - Y.y;
+      const float gy = xy ? Y.x - X.y : X.y + Y.x;
+      const float ex = xy ? X.y : X.x, ey = Y.y;
+      const float ey0 = xy ? ey : Y.x;
+      bh.st2(make_float2(c0 ? ex : gx, c0 ? ey0 : gy), off, 16u * TL * HN * m);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers for this N
 // ---------------------------------------------------------------------------
@@ -810,7 +978,10 @@ hipError_t NSK_CAT(launch_row_, NSK_N)(int kind, int scheme, const RowArgs& a, i
 namespace {
 template <int MODE>
 hipError_t go_col(const ColArgs& a, int batch, hipStream_t s) {
-  hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColK<NN_>::T), 0, s, a);
+  if constexpr (ColGeo<NN_>::XY)
+    hipLaunchKernelGGL((k_colxy<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColXY<NN_>::T), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 

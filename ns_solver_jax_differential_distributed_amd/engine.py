@@ -12,6 +12,7 @@ float32 (2, N, N) per slot.
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import Optional
 
 import numpy as np
@@ -158,7 +159,7 @@ class RolloutEngine:
         slots[:, 4] = steps
         nb = self.bands()
         gpart = self.buf("gpart", (self.batch * steps * nb * 2,), torch.float64)
-        seg = self.buf("seg", (self.batch, k, 2, self.N, self.N))
+        seg = self.buf("seg", (self.batch, k, 2, self.N, self.N)) if k > 1 else None
         dth = torch.empty((self.batch, 2), dtype=torch.float64, device=ckpt.device)
         du0 = torch.empty((self.batch, 2, self.N, self.N), dtype=torch.float32,
                           device=ckpt.device) if want_du0 else None
@@ -168,11 +169,27 @@ class RolloutEngine:
             _lib.OBJECTIVE[self.objective], 1.0, None if lf is None else lf.data_ptr(),
             dth.data_ptr(), None if du0 is None else du0.data_ptr(),
             *[self.spec(i).data_ptr() for i in range(4)], gpart.data_ptr(), ckpt.data_ptr(),
-            seg.data_ptr(), k, _lib.stream_ptr()), "rollout_adjoint")
+            None if seg is None else seg.data_ptr(), k, _lib.stream_ptr()), "rollout_adjoint")
         return dth, du0
 
-    def rollout_and_grad(self, u0, nu, dt, theta, steps, ckpt_every: int = 50):
-        """(J, u_final, dJ/dtheta (B,2), dJ/du0); ``steps`` int or (B,)."""
+    AUTO_CKPT_BYTES = 8 << 30
+
+    def resolve_ckpt(self, ckpt_every, steps: int) -> int:
+        """The checkpoint interval for ``ckpt_every``: an int as given, or for
+        "auto" (also None or 0) 1 - every state kept, the reverse sweep
+        recomputes nothing - when all slots' trajectories fit AUTO_CKPT_BYTES,
+        else ~sqrt(steps), which minimises checkpoints + segment."""
+        steps = max(int(steps), 1)
+        if ckpt_every not in (None, 0, "auto"):
+            return max(1, min(int(ckpt_every), steps))
+        if self.batch * steps * 8 * self.N * self.N <= self.AUTO_CKPT_BYTES:
+            return 1
+        return max(1, int(math.isqrt(steps)))
+
+    def rollout_and_grad(self, u0, nu, dt, theta, steps, ckpt_every=50):
+        """(J, u_final, dJ/dtheta (B,2), dJ/du0); ``steps`` int or (B,);
+        ``ckpt_every`` an int or "auto" (resolve_ckpt)."""
+        ckpt_every = self.resolve_ckpt(ckpt_every, int(self.step_counts(steps).max()))
         if int(self.step_counts(steps).max()) < 1:
             u0 = self.field(u0)
             z = torch.zeros((self.batch,), dtype=torch.float64, device=u0.device)
@@ -192,7 +209,8 @@ class RolloutFunction(torch.autograd.Function):
     the second get exact zeros (the forcing reads only theta[0], theta[1])."""
 
     @staticmethod
-    def forward(ctx, u0, theta, engine: RolloutEngine, nu, dt, steps: int, ckpt_every: int):
+    def forward(ctx, u0, theta, engine: RolloutEngine, nu, dt, steps: int, ckpt_every):
+        ckpt_every = engine.resolve_ckpt(ckpt_every, steps)
         J, uf, ckpt = engine.rollout(u0, nu, dt, theta, steps, ckpt_every if steps > 0 else None)
         ctx.engine, ctx.nu, ctx.dt, ctx.steps, ctx.k = engine, nu, dt, int(steps), ckpt_every
         ctx.theta_shape, ctx.theta_dtype = tuple(theta.shape), theta.dtype

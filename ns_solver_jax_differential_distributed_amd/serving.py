@@ -223,7 +223,7 @@ class _GradLane:
         ncheck = (steps + k - 1) // k
         return B * (8 * N * N * (4 + 1 + 1) + 8 * N * N * (ncheck + k) + 16 * steps * 256) + 1024
 
-    def launch(self, reqs: List[SolveRequest], now: float, nbytes: int) -> None:
+    def launch(self, reqs: List[SolveRequest], now: float, nbytes: int, ckpt_every: int) -> None:
         B = len(reqs)
         eng = self.engines.get(B)
         srv = self.srv
@@ -237,7 +237,7 @@ class _GradLane:
             nu = np.array([r.nu for r in reqs])
             dt = np.array([r.dt for r in reqs])
             steps = np.array([r.steps for r in reqs])
-            J, uf, dth, du0 = eng.rollout_and_grad(u0, nu, dt, th, steps, srv.grad_ckpt_every)
+            J, uf, dth, du0 = eng.rollout_and_grad(u0, nu, dt, th, steps, ckpt_every)
             ev = torch.cuda.Event()
             ev.record(self.stream)
         self.running = (ev, reqs, (J, uf, dth, du0), nbytes, now)
@@ -270,7 +270,7 @@ class NSServer:
 
     def __init__(self, L: float = 2 * math.pi, phys=(0.5, 0.1, 1.0), scheme: str = "central",
                  objective: str = "work", chunk: int = 16, max_steps: int = 512,
-                 capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 64 << 30,
+                 capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 96 << 30,
                  grad_group: int = 8, grad_ckpt_every: int = 25, use_graph: bool = True):
         if chunk < 2 or chunk % 2:
             raise ValueError("chunk must be an even number >= 2")
@@ -357,12 +357,18 @@ class NSServer:
             if lane.busy():
                 continue
             grp = [q.popleft() for _ in range(min(self.grad_group, len(q)))]
-            need = _GradLane.group_bytes(N, len(grp), max(r.steps for r in grp), self.grad_ckpt_every)
+            smax = max(r.steps for r in grp)
+            # every state checkpointed (no recomputation in the reverse sweep)
+            # when the group's trajectory fits half the budget
+            k = self.grad_ckpt_every
+            if _GradLane.group_bytes(N, len(grp), smax, 1) <= self.budget.limit // 2:
+                k = 1
+            need = _GradLane.group_bytes(N, len(grp), smax, k)
             if not self.budget.take(need):
                 for r in reversed(grp):
                     q.appendleft(r)
                 continue
-            lane.launch(grp, now, need)
+            lane.launch(grp, now, need, k)
 
     def step(self) -> None:
         """Admit waiting requests, advance every busy bucket by one chunk

@@ -137,6 +137,27 @@ def test_gradient_matches_adjoint_oracle(scheme, objective):
     assert abs(float(J) - Jo) / abs(Jo) < 1e-5
 
 
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+def test_gradient_without_recomputation_is_bitwise_equal(scheme):
+    """ckpt_every=1 (every state kept, reverse sweep reads the checkpoints)
+    and ckpt_every=5 (segments recomputed from checkpoints) give identical
+    bits: recomputed states are bit-identical to the forward's, and "auto"
+    picks the first for a trajectory this small."""
+    N, L, nu, dt, steps = 64, 1.0, 0.01, 1e-3, 17
+    rng = np.random.default_rng(7)
+    u0 = torch.from_numpy((0.4 * rng.standard_normal((2, N, N))).astype(np.float32))
+    theta = [0.8, 1.3, 0.2]
+    outs = []
+    for k in (1, 5, "auto"):
+        s = solver(N, L, nu, dt, steps, scheme, ckpt_every=k)
+        outs.append([t.cpu() for t in s.rollout_and_grad(u0, theta)])
+    assert s.engine.resolve_ckpt("auto", steps) == 1
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a, b)
+    for a, b in zip(outs[0], outs[2]):
+        assert torch.equal(a, b)
+
+
 def test_jit_gradient_matches_jax_reverse_mode():
     """JAX's gradient at theta=[1,0.5] from the reference's SGD(lr=0.1)
     trajectory (studies.py:890-918): g = (theta_0 - theta_1)/0.1;
