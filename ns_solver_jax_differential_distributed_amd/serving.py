@@ -32,6 +32,7 @@ import ctypes
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -120,6 +121,10 @@ class _Bucket:
             self.J = torch.zeros((capacity,), dtype=torch.float64, device=dev)
             rec = _lib.slot_records(0.0, 0.0, 0.0, 0.0, 0, 0, capacity)
             self.slots = torch.from_numpy(rec).to(dev)
+        # pinned staging rows for the slot records: admission copies them
+        # asynchronously on the bucket stream, behind any chunk in flight
+        self.stage = torch.zeros((capacity, _lib.SLOT_WORDS), dtype=torch.int32).pin_memory()
+        self.inflight: deque = deque()   # events of the chunks enqueued and not yet seen done
         self.req: List[Optional[SolveRequest]] = [None] * capacity
         self.step = np.zeros(capacity, dtype=np.int64)
         self.admit_t = np.zeros(capacity)
@@ -136,6 +141,11 @@ class _Bucket:
     def active(self) -> bool:
         return any(r is not None for r in self.req)
 
+    def chunks_in_flight(self) -> int:
+        while self.inflight and self.inflight[0].query():
+            self.inflight.popleft()
+        return len(self.inflight)
+
     def admit(self, req: SolveRequest, b: int, now: float) -> None:
         L = self.srv.L
         lib = _lib.lib()
@@ -143,7 +153,10 @@ class _Bucket:
             u0 = _u0_for(req, L)
             rec = _lib.slot_records(req.nu, req.dt, float(req.theta[0]), float(req.theta[1]), 0,
                                     req.steps, 1)
-            self.slots[b].copy_(torch.from_numpy(rec[0]), non_blocking=False)
+            # row b is rewritten only after its previous request finished,
+            # long after that request's copy ran
+            self.stage[b].numpy()[:] = rec[0]
+            self.slots[b].copy_(self.stage[b], non_blocking=True)
             st = self.stream.cuda_stream
             _lib.check(lib.nsk_admit(ctypes.byref(self.plan1), u0.data_ptr(), self.slots[b].data_ptr(),
                                      _lib.SCHEME[self.srv.scheme], _lib.OBJECTIVE[self.srv.objective],
@@ -187,6 +200,7 @@ class _Bucket:
             self.pending_done.append((ev, outs))
         self.last_event = torch.cuda.Event()
         self.last_event.record(self.stream)
+        self.inflight.append(self.last_event)
         return done
 
     def harvest(self, now_fn, results: List[SolveResult], block: bool) -> None:
@@ -207,13 +221,17 @@ class _Bucket:
 
 
 class _GradLane:
-    """Batched forward + adjoint for gradient requests of one grid size."""
+    """Batched forward + adjoint for gradient requests of one grid size.
+
+    The group's kernels (several per step, no graph) are enqueued by the
+    server's worker thread, so the serving loop keeps advancing the rollout
+    buckets meanwhile (the C calls release the GIL)."""
 
     def __init__(self, srv: "NSServer", N: int):
         self.srv, self.N = srv, N
         self.stream = torch.cuda.Stream()
         self.engines: Dict[int, RolloutEngine] = {}
-        self.running = None      # (event, reqs, outputs, bytes, admitted)
+        self.running = None      # (future -> (event, outputs), reqs, bytes, admitted)
 
     def busy(self) -> bool:
         return self.running is not None
@@ -223,7 +241,7 @@ class _GradLane:
         ncheck = (steps + k - 1) // k
         return B * (8 * N * N * (4 + 1 + 1) + 8 * N * N * (ncheck + k) + 16 * steps * 256) + 1024
 
-    def launch(self, reqs: List[SolveRequest], now: float, nbytes: int, ckpt_every: int) -> None:
+    def _enqueue(self, reqs: List[SolveRequest], ckpt_every: int):
         B = len(reqs)
         eng = self.engines.get(B)
         srv = self.srv
@@ -237,17 +255,29 @@ class _GradLane:
             nu = np.array([r.nu for r in reqs])
             dt = np.array([r.dt for r in reqs])
             steps = np.array([r.steps for r in reqs])
-            J, uf, dth, du0 = eng.rollout_and_grad(u0, nu, dt, th, steps, ckpt_every)
+            out = eng.rollout_and_grad(u0, nu, dt, th, steps, ckpt_every)
             ev = torch.cuda.Event()
             ev.record(self.stream)
-        self.running = (ev, reqs, (J, uf, dth, du0), nbytes, now)
+        return ev, out
+
+    def launch(self, reqs: List[SolveRequest], now: float, nbytes: int, ckpt_every: int) -> None:
+        fut = self.srv.worker.submit(self._enqueue, reqs, ckpt_every)
+        self.running = (fut, reqs, nbytes, now)
+
+    def wait(self) -> None:
+        if self.running is not None:
+            ev, _ = self.running[0].result()
+            ev.synchronize()
 
     def harvest(self, now_fn, results: List[SolveResult], block: bool) -> int:
         if self.running is None:
             return 0
-        ev, reqs, (J, uf, dth, du0), nbytes, t_adm = self.running
+        fut, reqs, nbytes, t_adm = self.running
         if block:
-            ev.synchronize()
+            self.wait()
+        if not fut.done():
+            return 0
+        ev, (J, uf, dth, du0) = fut.result()
         if not ev.query():
             return 0
         t = now_fn()
@@ -283,6 +313,7 @@ class NSServer:
         self.budget = HBMBudget(hbm_budget)
         self.grad_group, self.grad_ckpt_every = int(grad_group), int(grad_ckpt_every)
         self.use_graph = bool(use_graph)
+        self.worker = ThreadPoolExecutor(max_workers=1, thread_name_prefix="nsk-grad")
         self.buckets: Dict[int, _Bucket] = {}
         self.lanes: Dict[int, _GradLane] = {}
         self.queue: deque = deque()
@@ -324,7 +355,8 @@ class NSServer:
         if b is not None:
             return b
         cap = self.capacity.get(N, 4)
-        nb = 256 if N >= 2048 else max(1, N // 16)
+        p1 = _lib.make_plan(N, self.L, *self.phys, 1)
+        nb = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p1), 1))
         per = _Bucket.slot_bytes(N, self.max_steps, nb)
         while cap > 0 and not self.budget.take(per * cap):
             cap //= 2
@@ -371,26 +403,23 @@ class NSServer:
             lane.launch(grp, now, need, k)
 
     def step(self) -> None:
-        """Admit waiting requests, advance every busy bucket by one chunk
-        (each on its own stream), wait for those chunks (not for running
-        gradient groups) and harvest whatever has finished."""
+        """Admit waiting requests, keep every busy bucket two chunks ahead
+        (each bucket on its own stream, at its own pace: a small-N bucket is
+        not held to a large-N one's chunk time), and harvest whatever has
+        finished.  Never blocks on the GPU."""
         self._admit()
-        launched = []
-        for bk in self.buckets.values():
-            if bk.active():
-                bk.advance()
-                launched.append(bk)
-        for bk in launched:
-            bk.last_event.synchronize()
-        if not launched:
-            # only gradient groups in flight: wait for the first to finish
-            busy = [l for l in self.lanes.values() if l.busy()]
-            if busy and not self.queue:
-                busy[0].running[0].synchronize()
+        launched = False
         for bk in self.buckets.values():
             bk.harvest(self.now, self.results, False)
+            # two chunks in flight per bucket: the GPU does not wait for the
+            # host between chunks
+            if bk.active() and bk.chunks_in_flight() < 2:
+                bk.advance()
+                launched = True
         for lane in self.lanes.values():
             self.budget.give(lane.harvest(self.now, self.results, False))
+        if not launched:
+            time.sleep(5e-5)
 
     def run(self, requests: Sequence[SolveRequest], realtime: bool = True) -> List[SolveResult]:
         """Serve requests (their ``arrival`` fields are offsets from the start
