@@ -48,6 +48,7 @@ struct RowArgs {
   long long us_bstride;
   int us_base;
   float jseed;
+  int rows;               // rows per stream (set by the launcher)
 };
 
 struct ColArgs {
@@ -72,26 +73,42 @@ struct StencilArgs {
 };
 
 // Row-pass geometry for grid size N.  A workgroup runs LG row streams of
-// N/EE threads each (EE = min(E, N) points per thread); stream g walks ROWS
-// consecutive rows.  DB: double-buffered LDS exchanges.
+// N/EE threads each (EE = min(E, N) points per thread); stream g walks
+// `rows` consecutive rows, chosen per launch in [RMIN, RMAX] from the batch
+// size (pick_rows): short bands give more workgroups to a small batch,
+// long ones re-transform fewer halo rows when the batch fills the GPU.
+// DB: double-buffered LDS exchanges.
 #ifndef NSK_ROW_E4096
 #define NSK_ROW_E4096 16
 #endif
-#ifndef NSK_ROW_ROWS4096
-#define NSK_ROW_ROWS4096 8
-#endif
 template <int N>
 struct RowGeo;
-template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, ROWS = NSK_ROW_ROWS4096; static constexpr bool DB = true; };
-template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<256> { static constexpr int E = 16, LG = 4, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<128> { static constexpr int E = 16, LG = 8, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<64> { static constexpr int E = 16, LG = 16, ROWS = 4; static constexpr bool DB = true; };
-template <> struct RowGeo<32> { static constexpr int E = 16, LG = 16, ROWS = 2; static constexpr bool DB = true; };
-template <> struct RowGeo<16> { static constexpr int E = 16, LG = 16, ROWS = 1; static constexpr bool DB = true; };
-template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, ROWS = 1; static constexpr bool DB = true; };
+template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = 8, RMAX = 16; static constexpr bool DB = true; };
+template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
+template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
+template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
+template <> struct RowGeo<256> { static constexpr int E = 16, LG = 4, RMIN = 2, RMAX = 8; static constexpr bool DB = true; };
+template <> struct RowGeo<128> { static constexpr int E = 16, LG = 8, RMIN = 2, RMAX = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<64> { static constexpr int E = 16, LG = 16, RMIN = 2, RMAX = 4; static constexpr bool DB = true; };
+template <> struct RowGeo<32> { static constexpr int E = 16, LG = 16, RMIN = 2, RMAX = 2; static constexpr bool DB = true; };
+template <> struct RowGeo<16> { static constexpr int E = 16, LG = 16, RMIN = 1, RMAX = 1; static constexpr bool DB = true; };
+template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, RMIN = 1, RMAX = 1; static constexpr bool DB = true; };
+
+// Rows per stream for a launch over `batch` slots: enough workgroups to
+// fill the GPU (about 2048 resident waves; 1024 below N = 1024, where a
+// wave holds several streams), as few halo rows as that allows.
+template <int N>
+constexpr int pick_rows(int batch) {
+  using G = RowGeo<N>;
+  constexpr int EE = N < G::E ? N : G::E;
+  constexpr int TL = N / EE;
+  const long target = N >= 1024 ? 2048 : 1024;
+  // waves = batch * (N / rows) * TL / 64
+  const long num = (long)batch * N * TL;
+  int rows = G::RMIN;
+  while (rows * 2 <= G::RMAX && num / (64L * rows * 2) >= target) rows *= 2;
+  return rows;
+}
 
 // Column-pass geometry: W column pairs (lines X and Y each) per block, E
 // points per thread, NWG workgroups per slot (each loops over blocks NWG
@@ -114,9 +131,18 @@ struct ColGeo;
 #define NSK_COL_XY4096 0
 #endif
 template <> struct ColGeo<4096> { static constexpr int E = NSK_COL_E4096, W = NSK_COL_W4096, NWG = NSK_COL_NWG4096; static constexpr bool PF = NSK_COL_PF, XY = NSK_COL_XY4096; };
-template <> struct ColGeo<2048> { static constexpr int E = 16, W = 2, NWG = 512; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<1024> { static constexpr int E = 16, W = 4, NWG = 128; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<512> { static constexpr int E = 16, W = 8, NWG = 32; static constexpr bool PF = false, XY = false; };
+#ifndef NSK_COL_W2048
+#define NSK_COL_W2048 2
+#endif
+#ifndef NSK_COL_W1024
+#define NSK_COL_W1024 4
+#endif
+#ifndef NSK_COL_W512
+#define NSK_COL_W512 8
+#endif
+template <> struct ColGeo<2048> { static constexpr int E = 16, W = NSK_COL_W2048, NWG = 1024 / NSK_COL_W2048; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<1024> { static constexpr int E = 16, W = NSK_COL_W1024, NWG = 512 / NSK_COL_W1024; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<512> { static constexpr int E = 16, W = NSK_COL_W512, NWG = 256 / NSK_COL_W512; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<256> { static constexpr int E = 16, W = 16, NWG = 8; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<128> { static constexpr int E = 16, W = 32, NWG = 2; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<64> { static constexpr int E = 16, W = 32, NWG = 1; static constexpr bool PF = false, XY = false; };
@@ -124,8 +150,12 @@ template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; st
 template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<8> { static constexpr int E = 16, W = 4, NWG = 1; static constexpr bool PF = false, XY = false; };
 
+// Objective / gradient partial sums per slot and step: one per band of
+// RMIN rows (a workgroup over longer bands writes its sum into the first of
+// its entries and zeros into the others), so the layout does not depend on
+// the launch's batch size.
 template <int N>
-constexpr int row_bands_c() { return N / (RowGeo<N>::LG * RowGeo<N>::ROWS); }
+constexpr int row_bands_c() { return N / (RowGeo<N>::LG * RowGeo<N>::RMIN); }
 
 inline int row_bands(int N) {
   switch (N) {

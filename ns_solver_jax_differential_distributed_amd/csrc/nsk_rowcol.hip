@@ -252,7 +252,7 @@ template <int N>
 struct RowK {
   using G = RowGeo<N>;
   using F = RF<N, G::E>;
-  static constexpr int EE = F::EE, TL = F::T, LG = G::LG, ROWS = G::ROWS;
+  static constexpr int EE = F::EE, TL = F::T, LG = G::LG, RMIN = G::RMIN, RMAX = G::RMAX;
   static constexpr int T = LG * TL;
   static constexpr int NW = TL >= 64 ? TL / 64 : 0;  // waves per row stream
   static constexpr int LS = line_stride(LineLen<N>::NP, TL < 16 ? TL : 0);
@@ -260,18 +260,18 @@ struct RowK {
 };
 
 // The row pass.  Workgroup `band` runs LG row streams; stream g walks the
-// ROWS rows x = xs .. xs+ROWS-1.  For the stepping kinds each thread keeps
-// its points (y = t + TL*m) of rows x-1, x and x+1 in registers (rp, rc,
-// rn); the y-neighbours of row x come from the adjacent lanes (shuffles),
-// and across wave boundaries from an LDS edge table written right after
-// the row was produced (two parities, so that row x+1's edges never
-// overwrite row x's before they are read; at least one barrier separates
-// every write from its reads).
+// `rows` rows x = xs .. xs+rows-1 (chosen per launch, pick_rows).  For the
+// stepping kinds each thread keeps its points (y = t + TL*m) of rows x-1, x
+// and x+1 in registers (rp, rc, rn); the y-neighbours of row x come from the
+// adjacent lanes (shuffles), and across wave boundaries from an LDS edge
+// table written right after the row was produced (two parities, so that row
+// x+1's edges never overwrite row x's before they are read; at least one
+// barrier separates every write from its reads).
 template <int N, int KIND, int SCHEME>
 __global__ void __launch_bounds__(RowK<N>::T)
     __attribute__((amdgpu_waves_per_eu(RowK<N>::WPE))) k_row(RowArgs a) {
   using K = RowK<N>;
-  constexpr int E = RowGeo<N>::E, EE = K::EE, TL = K::TL, LG = K::LG, ROWS = K::ROWS, T = K::T;
+  constexpr int E = RowGeo<N>::E, EE = K::EE, TL = K::TL, LG = K::LG, RMAX = K::RMAX, T = K::T;
   constexpr int NW = K::NW, LS = K::LS, HN = N / 2;
   constexpr bool DB = RowGeo<N>::DB;
   constexpr int SRC = (KIND == RK_REAL_TO_SPEC || KIND == RK_STEP_FROM_REAL || KIND == RK_STEP_FORCE)
@@ -284,8 +284,8 @@ __global__ void __launch_bounds__(RowK<N>::T)
   static_assert(NW == 0 || RF<N, E>::S > 1, "edge table needs the exchange barriers");
   __shared__ float2 s_x[(DB ? 2 : 1) * LG * LS];
   __shared__ float2 s_edge[NW > 0 ? 2 * NW * 2 * EE : 1];  // [parity][wave][first,last][m]
-  __shared__ float s_f[LG * ROWS];
-  __shared__ double s_dA[OP == OP_ADJ ? LG * ROWS : 1], s_dB[OP == OP_ADJ ? LG * ROWS : 1];
+  __shared__ float s_f[LG * RMAX];
+  __shared__ double s_dA[OP == OP_ADJ ? LG * RMAX : 1], s_dB[OP == OP_ADJ ? LG * RMAX : 1];
   __shared__ double s_red[T >= 64 ? T / 64 : T];
   __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
 
@@ -322,7 +322,8 @@ __global__ void __launch_bounds__(RowK<N>::T)
   const bool two = a.src_nc > 1;
   const bool two_out = a.dst_nc > 1;
 
-  const int xs = (band * LG + g) * ROWS;  // first row of this stream
+  const int rows = a.rows;
+  const int xs = (band * LG + g) * rows;  // first row of this stream
 
   Twid<N, E> tw;
   tw.load(t, s_tw, tid, T);
@@ -378,7 +379,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
   if (OP == OP_COPY || (ROLL && final_mode)) {
     __syncthreads();  // twiddle table
 #pragma unroll 1
-    for (int i = 0; i < ROWS; ++i) {
+    for (int i = 0; i < rows; ++i) {
       const int x = xs + i;
       float2 v[1][EE];
       if constexpr (SRC == SRC_SPEC) {
@@ -395,8 +396,8 @@ __global__ void __launch_bounds__(RowK<N>::T)
   }
 
   // ---- per-row forcing (f_x depends on x only, f_y = 0)
-  if (tid < LG * ROWS) {
-    const int x = band * LG * ROWS + tid;
+  if (tid < LG * rows) {
+    const int x = band * LG * rows + tid;
     const float xc = p.Lf * ((float)x / (float)N);
     s_f[tid] = FFIELD ? 0.0f : forcing_x(sl.A, sl.B, tm, xc, p);
     if constexpr (OP == OP_ADJ) {
@@ -488,12 +489,12 @@ __global__ void __launch_bounds__(RowK<N>::T)
   fetch_row(rc, xs);
   publish(rc, xs);
 #pragma unroll 1
-  for (int i = 0; i < ROWS; ++i) {
+  for (int i = 0; i < rows; ++i) {
     const int x = xs + i;
     fetch_row(rn, x + 1);
     publish(rn, x + 1);
     if constexpr (SRC == SRC_REAL && NW > 0) __syncthreads();
-    const int r = g * ROWS + i;
+    const int r = g * rows + i;
     if constexpr (OP == OP_STEP) {
       // checkpoint of u_n and the objective's term before the step (one
       // uniform branch each, outside the per-point loop)
@@ -577,14 +578,18 @@ __global__ void __launch_bounds__(RowK<N>::T)
     }
   }
 
+  // partial sums: one entry per band of RMIN rows (row_bands_c); this
+  // workgroup's rows*LG rows span `span` of them
+  constexpr int NB = row_bands_c<N>();
+  const int span = rows / K::RMIN;
   if constexpr (OP == OP_STEP && !FFIELD) {
     if (a.part != nullptr) {
       const double s = block_sum<T>(acc0, s_red, tid);
-      if (tid == 0 && n < a.pcap) {
+      if (tid < span && n < a.pcap) {
         const double scale = (a.objective == NSK_OBJ_WORK)
                                  ? (double)dt * dx2
                                  : 1.0 / ((double)sl.nsteps * (double)NN);
-        a.part[((size_t)b * a.pcap + n) * nb + band] = s * scale;
+        a.part[((size_t)b * a.pcap + n) * NB + band * span + tid] = (tid == 0) ? s * scale : 0.0;
       }
     }
   }
@@ -592,10 +597,10 @@ __global__ void __launch_bounds__(RowK<N>::T)
     if (a.part != nullptr) {
       const double sA = block_sum<T>(acc0, s_red, tid);
       const double sB = block_sum<T>(acc1, s_red, tid);
-      if (tid == 0 && n < a.pcap) {
-        double* q = a.part + (((size_t)b * a.pcap + n) * nb + band) * 2;
-        q[0] = sA;
-        q[1] = sB;
+      if (tid < span && n < a.pcap) {
+        double* q = a.part + (((size_t)b * a.pcap + n) * NB + band * span + tid) * 2;
+        q[0] = (tid == 0) ? sA : 0.0;
+        q[1] = (tid == 0) ? sB : 0.0;
       }
     }
   }
@@ -949,8 +954,10 @@ constexpr int NN_ = NSK_N;
 
 namespace {
 template <int KIND, int SCHEME>
-hipError_t go_row(const RowArgs& a, int batch, hipStream_t s) {
-  const int bands = row_bands_c<NN_>();
+hipError_t go_row(const RowArgs& a0, int batch, hipStream_t s) {
+  RowArgs a = a0;
+  a.rows = pick_rows<NN_>(batch);
+  const int bands = NN_ / (RowGeo<NN_>::LG * a.rows);
   hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(RowK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
