@@ -204,19 +204,20 @@ __global__ void k_stencil(StencilArgs a) {
 
 // out[b*ncomp + c] = sum_{n < nsteps_b, band} part[b][n][band][c], in a
 // fixed order (thread-strided partial sums, then a fixed tree).
-__global__ void k_finalize(const double* part, int pcap, int nb, int ncomp,
-                           const nsk_slot* slots, double* out) {
-  __shared__ double red[256];
+constexpr int kFinT = 1024;
+__global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap, int nb, int ncomp,
+                                                    const nsk_slot* slots, double* out) {
+  __shared__ double red[kFinT];
   const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
   int nsteps = slots[b].nsteps;
   if (nsteps > pcap) nsteps = pcap;
   const size_t total = (size_t)nsteps * nb;
   const double* base = part + (size_t)b * pcap * nb * ncomp;
   double s = 0.0;
-  for (size_t k = tid; k < total; k += 256) s += base[k * ncomp + c];
+  for (size_t k = tid; k < total; k += kFinT) s += base[k * ncomp + c];
   red[tid] = s;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = kFinT / 2; w > 0; w >>= 1) {
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
@@ -261,7 +262,7 @@ hipError_t zero_spec(const nsk_plan* p, void* spec, hipStream_t st) {
 }
 hipError_t finalize(const nsk_plan* p, const nsk_slot* slots, const double* part, int pcap,
                     int ncomp, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(p->batch, ncomp), dim3(256), 0, st, part, pcap,
+  hipLaunchKernelGGL(k_finalize, dim3(p->batch, ncomp), dim3(kFinT), 0, st, part, pcap,
                      row_bands(p->N), ncomp, slots, out);
   return hipGetLastError();
 }
