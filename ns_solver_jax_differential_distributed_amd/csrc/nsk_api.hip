@@ -263,7 +263,7 @@ hipError_t zero_spec(const nsk_plan* p, void* spec, hipStream_t st) {
 hipError_t finalize(const nsk_plan* p, const nsk_slot* slots, const double* part, int pcap,
                     int ncomp, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(p->batch, ncomp), dim3(kFinT), 0, st, part, pcap,
-                     row_bands(p->N), ncomp, slots, out);
+                     row_parts(p->N), ncomp, slots, out);
   return hipGetLastError();
 }
 
@@ -454,7 +454,7 @@ size_t nsk_spectrum_bytes(const nsk_plan* p) {
 }
 
 size_t nsk_partials_per_slot(const nsk_plan* p, int steps) {
-  return p ? (size_t)(steps > 0 ? steps : 0) * row_bands(p->N) : 0;
+  return p ? (size_t)(steps > 0 ? steps : 0) * row_parts(p->N) : 0;
 }
 
 int nsk_grad(const nsk_plan* p, const float* f, float* gx, float* gy, void* stream) {

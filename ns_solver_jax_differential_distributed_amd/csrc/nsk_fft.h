@@ -49,8 +49,12 @@ __device__ float2 g_tw[TW_N];
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Products written with explicit fused multiply-adds: the FFT then has no
+// multiply/add pair left for the compiler to contract, so every inlined copy
+// of a transform rounds identically (a row transformed in a loop prologue
+// equals the same row transformed inside the loop).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 
 // cos / sin of 2*pi*i/16, i = 0..15 (non-recursive so that it folds to a
@@ -86,7 +90,7 @@ __device__ __forceinline__ float2 rot_const(int q, float2 v) {
   if (idx == 4) return DIR < 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
   const float c = w16c(idx);
   const float s = DIR < 0 ? -w16s(idx) : w16s(idx);
-  return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+  return make_float2(fmaf(v.x, c, -(v.y * s)), fmaf(v.x, s, v.y * c));
 }
 
 // In-register R-point DFT, natural order in and out (radix-2 decimation in

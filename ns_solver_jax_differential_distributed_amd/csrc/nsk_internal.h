@@ -150,25 +150,31 @@ template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; st
 template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<8> { static constexpr int E = 16, W = 4, NWG = 1; static constexpr bool PF = false, XY = false; };
 
-// Objective / gradient partial sums per slot and step: one per band of
-// RMIN rows (a workgroup over longer bands writes its sum into the first of
-// its entries and zeros into the others), so the layout does not depend on
-// the launch's batch size.
+// Objective / gradient partial sums per slot and step: one entry per RMIN
+// rows of one wave's share of a row stream (entry (x / RMIN) * NWS + w for
+// row x and wave w of the stream, NWS waves per stream).  Every thread sums
+// its points over the RMIN rows in order, the wave reduces by a fixed
+// shuffle tree, so neither the entries nor their values depend on the band
+// length a launch picks for its batch size.
 template <int N>
-constexpr int row_bands_c() { return N / (RowGeo<N>::LG * RowGeo<N>::RMIN); }
+constexpr int row_parts_c() {
+  constexpr int EE = N < RowGeo<N>::E ? N : RowGeo<N>::E;
+  constexpr int TL = N / EE;
+  return (N / RowGeo<N>::RMIN) * (TL >= 64 ? TL / 64 : 1);
+}
 
-inline int row_bands(int N) {
+inline int row_parts(int N) {
   switch (N) {
-    case 4096: return row_bands_c<4096>();
-    case 2048: return row_bands_c<2048>();
-    case 1024: return row_bands_c<1024>();
-    case 512: return row_bands_c<512>();
-    case 256: return row_bands_c<256>();
-    case 128: return row_bands_c<128>();
-    case 64: return row_bands_c<64>();
-    case 32: return row_bands_c<32>();
-    case 16: return row_bands_c<16>();
-    case 8: return row_bands_c<8>();
+    case 4096: return row_parts_c<4096>();
+    case 2048: return row_parts_c<2048>();
+    case 1024: return row_parts_c<1024>();
+    case 512: return row_parts_c<512>();
+    case 256: return row_parts_c<256>();
+    case 128: return row_parts_c<128>();
+    case 64: return row_parts_c<64>();
+    case 32: return row_parts_c<32>();
+    case 16: return row_parts_c<16>();
+    case 8: return row_parts_c<8>();
     default: return 1;
   }
 }

@@ -92,31 +92,6 @@ __device__ inline void forcing_dtheta(float A, float B, float t, float x,
   dB = f * d * d / (sig * sig * sig) * (-pi2 * (double)t * sin(2.0 * bb));
 }
 
-// Deterministic workgroup sum (fixed shuffle tree per wave, then the waves
-// in order; one slot per thread below a full wave); the result is valid in
-// thread 0.
-template <int T>
-__device__ __forceinline__ double block_sum(double v, double* red, int tid) {
-  double s = 0.0;
-  if constexpr (T >= 64) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-      for (int w = 0; w < T / 64; ++w) s += red[w];
-    }
-  } else {
-    red[tid] = v;
-    __syncthreads();
-    if (tid == 0) {
-      for (int w = 0; w < T; ++w) s += red[w];
-    }
-  }
-  __syncthreads();
-  return s;
-}
-
 // Predictor u* = u + dt (nu lap u + adv(u) + f) at one cell from the packed
 // pair (ux + i uy) at the centre c and the neighbours x-1 (l), x+1 (r),
 // y-1 (d), y+1 (u).  Stencils of NS_solver.py:243-265, 474-477, 685-689.
@@ -286,7 +261,6 @@ __global__ void __launch_bounds__(RowK<N>::T)
   __shared__ float2 s_edge[NW > 0 ? 2 * NW * 2 * EE : 1];  // [parity][wave][first,last][m]
   __shared__ float s_f[LG * RMAX];
   __shared__ double s_dA[OP == OP_ADJ ? LG * RMAX : 1], s_dB[OP == OP_ADJ ? LG * RMAX : 1];
-  __shared__ double s_red[T >= 64 ? T / 64 : T];
   __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
 
   const int tid = threadIdx.x;
@@ -483,7 +457,42 @@ __global__ void __launch_bounds__(RowK<N>::T)
     }
   };
 
+  // Objective / gradient partial sums: each thread accumulates its points
+  // over RMIN rows, then the wave (or the lanes of a row stream narrower
+  // than a wave) reduces by a fixed shuffle tree and writes entry
+  // (x0 / RMIN) * NWS + w (x0: first of the RMIN rows, w: wave within the
+  // stream).  No barrier; the value does not depend on `rows`.
+  constexpr int RMIN = K::RMIN, NPART = row_parts_c<N>();
+  constexpr int NWS = TL >= 64 ? TL / 64 : 1, SEG = TL < 64 ? TL : 64;
+  constexpr bool HASPART = (OP == OP_STEP && !FFIELD) || OP == OP_ADJ;
+  double* const part = (HASPART && a.part != nullptr && n < a.pcap)
+                           ? a.part + ((size_t)b * a.pcap + n) * NPART * (OP == OP_ADJ ? 2 : 1)
+                           : nullptr;
+  const double pscale = (OP != OP_STEP) ? 1.0
+                        : (a.objective == NSK_OBJ_WORK ? (double)dt * dx2
+                                                       : 1.0 / ((double)sl.nsteps * (double)NN));
   double acc0 = 0.0, acc1 = 0.0;
+  auto flush = [&](int x0) {
+    if constexpr (HASPART) {
+      double v0 = acc0, v1 = acc1;
+#pragma unroll
+      for (int o = SEG / 2; o > 0; o >>= 1) {
+        v0 += __shfl_xor(v0, o);
+        if constexpr (OP == OP_ADJ) v1 += __shfl_xor(v1, o);
+      }
+      if (part != nullptr && (lane & (SEG - 1)) == 0) {
+        const int e = (x0 / RMIN) * NWS + (TL >= 64 ? (t >> 6) : 0);
+        if constexpr (OP == OP_ADJ) {
+          part[2 * e] = v0;
+          part[2 * e + 1] = v1;
+        } else {
+          part[e] = v0 * pscale;
+        }
+      }
+      acc0 = 0.0;
+      acc1 = 0.0;
+    }
+  };
   float2 rp[EE], rc[EE], rn[EE];
   fetch_row(rp, xs - 1);
   fetch_row(rc, xs);
@@ -576,33 +585,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
       rp[m] = rc[m];
       rc[m] = rn[m];
     }
-  }
-
-  // partial sums: one entry per band of RMIN rows (row_bands_c); this
-  // workgroup's rows*LG rows span `span` of them
-  constexpr int NB = row_bands_c<N>();
-  const int span = rows / K::RMIN;
-  if constexpr (OP == OP_STEP && !FFIELD) {
-    if (a.part != nullptr) {
-      const double s = block_sum<T>(acc0, s_red, tid);
-      if (tid < span && n < a.pcap) {
-        const double scale = (a.objective == NSK_OBJ_WORK)
-                                 ? (double)dt * dx2
-                                 : 1.0 / ((double)sl.nsteps * (double)NN);
-        a.part[((size_t)b * a.pcap + n) * NB + band * span + tid] = (tid == 0) ? s * scale : 0.0;
-      }
-    }
-  }
-  if constexpr (OP == OP_ADJ) {
-    if (a.part != nullptr) {
-      const double sA = block_sum<T>(acc0, s_red, tid);
-      const double sB = block_sum<T>(acc1, s_red, tid);
-      if (tid < span && n < a.pcap) {
-        double* q = a.part + (((size_t)b * a.pcap + n) * NB + band * span + tid) * 2;
-        q[0] = (tid == 0) ? sA : 0.0;
-        q[1] = (tid == 0) ? sB : 0.0;
-      }
-    }
+    if (((i + 1) & (RMIN - 1)) == 0) flush(x + 1 - RMIN);
   }
 }
 
