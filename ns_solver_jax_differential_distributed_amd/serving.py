@@ -16,7 +16,9 @@ work and harvests results.
 * Gradient requests are grouped per N (up to ``grad_group`` of them that
   are waiting when the lane is free) and run as one batched forward +
   discrete adjoint with per-slot step counts (``nsk_rollout`` with
-  checkpoints, then ``nsk_rollout_adjoint``).
+  checkpoints, then ``nsk_rollout_adjoint``).  Large groups matter: a group
+  of a few N = 128 slots is a launch of a few workgroups per kernel, and
+  one lane runs its groups one after another.
 * Every bucket and gradient lane has its own HIP stream (all at the default
   priority), so work of different sizes overlaps on the device.
 * Admission is bounded by an HBM budget: bucket tables and gradient
@@ -301,7 +303,7 @@ class NSServer:
     def __init__(self, L: float = 2 * math.pi, phys=(0.5, 0.1, 1.0), scheme: str = "central",
                  objective: str = "work", chunk: int = 16, max_steps: int = 512,
                  capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 96 << 30,
-                 grad_group: int = 8, grad_ckpt_every: int = 25, use_graph: bool = True):
+                 grad_group: int = 64, grad_ckpt_every: int = 25, use_graph: bool = True):
         if chunk < 2 or chunk % 2:
             raise ValueError("chunk must be an even number >= 2")
         if scheme not in _lib.SCHEME or objective not in _lib.OBJECTIVE:
