@@ -531,8 +531,9 @@ class SingleCoreNSSolver:
         self.objective_kind = objective
         self.use_graph = bool(use_graph)
         self._engine = None
-        self.jit_objective = self.objective
-        self.jit_gradient = self.gradient
+        self._batch_engines = {}
+        self.jit_objective = _Batchable(self.objective, self.objective_batch)
+        self.jit_gradient = _Batchable(self.gradient, self.gradient_batch, self.value_and_gradient_batch)
 
     @property
     def engine(self):
@@ -600,6 +601,72 @@ class SingleCoreNSSolver:
         u0 = torch.zeros((2, self.grid.N, self.grid.N), dtype=torch.float32, device=_device())
         _, _, g, _ = self.rollout_and_grad(u0, theta)
         return g
+
+    # ---- many thetas at once (not in the reference: callers such as
+    # studies.GradientBenchmark hand a whole finite-difference sweep or
+    # several optimizer runs to one batched launch sequence)
+    def _engine_for(self, B: int):
+        if B == 1:
+            return self.engine
+        eng = self._batch_engines.get(B)
+        if eng is None:
+            from .engine import RolloutEngine
+            eng = RolloutEngine(self.grid.N, self.grid.L, B, self.phys.mu0, self.phys.sigma0,
+                                self.phys.C, self.scheme, self.objective_kind, self.use_graph)
+            self._batch_engines[B] = eng
+        return eng
+
+    @staticmethod
+    def _theta_rows(thetas) -> np.ndarray:
+        th = np.asarray(thetas.detach().cpu() if isinstance(thetas, torch.Tensor) else thetas,
+                        dtype=np.float32)
+        if th.ndim != 2 or th.shape[0] < 1 or th.shape[1] < 2:
+            raise ValueError(f"thetas must be (B, >=2), got shape {th.shape}")
+        return th
+
+    def objective_batch(self, thetas):
+        """J(theta_b) from the zero initial condition for every row of thetas
+        (B, >=2), as one batched rollout; (B,) float64, entry b bit-identical
+        to objective(thetas[b])."""
+        th = self._theta_rows(thetas)
+        B, N = th.shape[0], self.grid.N
+        u0 = torch.zeros((B, 2, N, N), dtype=torch.float32, device=_device())
+        J, _, _ = self._engine_for(B).rollout(u0, self.phys.nu, self.time.dt, th[:, :2],
+                                              int(self.time.steps))
+        return J
+
+    def value_and_gradient_batch(self, thetas):
+        """(J (B,), dJ/dtheta (B, len theta)) from the zero initial condition
+        for every row of thetas, as one batched rollout + adjoint; row b
+        equals (objective(thetas[b]), gradient(thetas[b])) bit for bit."""
+        th = self._theta_rows(thetas)
+        B, N = th.shape[0], self.grid.N
+        u0 = torch.zeros((B, 2, N, N), dtype=torch.float32, device=_device())
+        J, _, dth, _ = self._engine_for(B).rollout_and_grad(u0, self.phys.nu, self.time.dt, th[:, :2],
+                                                           int(self.time.steps), self.ckpt_every)
+        g = torch.zeros((B, th.shape[1]), dtype=torch.float64, device=J.device)
+        g[:, :2] = dth
+        return J, g
+
+    def gradient_batch(self, thetas):
+        """dJ/dtheta for every row of thetas (B, >=2): (B, len theta) float64."""
+        return self.value_and_gradient_batch(thetas)[1]
+
+
+class _Batchable:
+    """A callable of one theta (the reference's ``jit_objective`` /
+    ``jit_gradient`` contract) that also offers ``batch(thetas)`` for many
+    thetas in one launch sequence and, for gradients,
+    ``value_and_grad_batch(thetas)``."""
+
+    def __init__(self, one, many, value_and_many=None):
+        self._one = one
+        self.batch = many
+        if value_and_many is not None:
+            self.value_and_grad_batch = value_and_many
+
+    def __call__(self, theta):
+        return self._one(theta)
 
 
 class BatchedNSSolver:

@@ -40,6 +40,17 @@ def main():
         "objectives": opt["SGD (lr=0.1)_objectives"][:6].astype(float).tolist(),
         "lr": 0.1,
     }
+    # all four optimizer runs of the optimization study (studies.py:864-947,
+    # 1120-1127): default config, theta0 = [1, 0.5], 100 steps each
+    out["optimization_study"] = {}
+    for name in ("Adam (lr=0.01)", "Adam (lr=0.001)", "SGD (lr=0.1)", "RMSprop (lr=0.01)"):
+        out["optimization_study"][name] = {
+            "parameters": opt[f"{name}_parameters"].astype(float).tolist(),
+            "objectives": opt[f"{name}_objectives"].astype(float).tolist(),
+            "obj_initial": float(opt[f"{name}_obj_initial"]),
+            "obj_final": float(opt[f"{name}_obj_final"]),
+            "improvement": float(opt[f"{name}_improvement"]),
+        }
     gb = np.load(os.path.join(res, "gradient_benchmark_results.npz"), allow_pickle=False)
     out["gradient_benchmark"] = {
         k: (gb[k].astype(float).tolist() if gb[k].shape else float(gb[k]))
