@@ -163,26 +163,29 @@ int nsk_rollout(const nsk_plan *p, const float *u0, float *u_final,
  * ckpt_every == 1: the reverse sweep then reads every u_n from ckpt and
  * recomputes nothing); gpart:
  * 2*nsk_partials_per_slot(p, steps) doubles per slot.  slots[b].step must
- * be 0 on entry and is `steps` on return. */
+ * be 0 on entry and is `steps` on return.  use_graph != 0 replays the
+ * forward chunks and, with ckpt_every == 1, the reverse sweep in chunks of
+ * 32 steps from HIP graphs (captured on first use, cached per buffer set). */
 int nsk_rollout_grad(const nsk_plan *p, const float *u0, float *u_final,
                      nsk_slot *slots, int steps, int scheme, int objective,
                      double *J_out, double *dtheta_out, float *du0_out,
                      void *spec0, void *spec1, void *spec2, void *spec3,
                      double *jpart, double *gpart, float *ckpt, float *seg,
-                     int ckpt_every, void *stream);
+                     int ckpt_every, int use_graph, void *stream);
 
 /* The adjoint sweep alone, from the checkpoints a previous nsk_rollout
  * stored (same slots, steps, scheme, objective, ckpt_every; slots[b].step is
  * `steps` on entry and on return).  Seeds: jseed = dL/dJ (the objective's
  * terms are scaled by it) and lam_final (nullable) = dL/du_final as (2,N,N)
  * per slot.  Outputs dtheta_out (nullable, 2 doubles per slot) and du0_out
- * (nullable).  spec0..spec3, gpart and seg as for nsk_rollout_grad. */
+ * (nullable).  spec0..spec3, gpart, seg and use_graph as for
+ * nsk_rollout_grad. */
 int nsk_rollout_adjoint(const nsk_plan *p, nsk_slot *slots, int steps, int scheme,
                         int objective, float jseed, const float *lam_final,
                         double *dtheta_out, float *du0_out, void *spec0,
                         void *spec1, void *spec2, void *spec3, double *gpart,
                         const float *ckpt, float *seg, int ckpt_every,
-                        void *stream);
+                        int use_graph, void *stream);
 
 /* ---- continuous batching building blocks ------------------------------ */
 /* Enters p->batch consecutive slots (pointers already offset to the first

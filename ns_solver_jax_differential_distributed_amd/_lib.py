@@ -71,14 +71,14 @@ _SIGS = {
     "nsk_step": ([_P, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp], _i),
     "nsk_rollout": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
     "nsk_rollout_grad": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                          _vp, _vp, _vp, _vp, _i, _vp], _i),
+                          _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
     "nsk_admit": ([_P, _vp, _vp, _i, _i, _vp, _vp, _i, _vp], _i),
     "nsk_advance": ([_P, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
     "nsk_finalize": ([_P, _vp, _vp, _i, _i, _vp, _vp], _i),
     "nsk_column_pass": ([_P, _vp, _i, _vp], _i),
     "nsk_row_pass": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp], _i),
     "nsk_rollout_adjoint": ([_P, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                             _vp, _vp, _i, _vp], _i),
+                             _vp, _vp, _i, _i, _vp], _i),
 }
 
 _lib = None

@@ -330,35 +330,28 @@ hipError_t enqueue_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int sche
 }
 
 struct GraphKey {
+  int kind;  // 0: rollout chunk, 1: reverse-sweep chunk
   int dev, N, batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap;
   const void *slots, *a, *b, *uf, *jp, *ck;
-  double L, mu0, sigma0, C;
+  double L, mu0, sigma0, C, jseed;
   bool operator<(const GraphKey& o) const {
-    return std::tie(dev, N, batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap, slots, a, b,
-                    uf, jp, ck, L, mu0, sigma0, C) <
-           std::tie(o.dev, o.N, o.batch, o.chunk, o.scheme, o.objective, o.jcap, o.ckpt_every,
-                    o.ckpt_cap, o.slots, o.a, o.b, o.uf, o.jp, o.ck, o.L, o.mu0, o.sigma0, o.C);
+    return std::tie(kind, dev, N, batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap, slots, a,
+                    b, uf, jp, ck, L, mu0, sigma0, C, jseed) <
+           std::tie(o.kind, o.dev, o.N, o.batch, o.chunk, o.scheme, o.objective, o.jcap, o.ckpt_every,
+                    o.ckpt_cap, o.slots, o.a, o.b, o.uf, o.jp, o.ck, o.L, o.mu0, o.sigma0, o.C, o.jseed);
   }
 };
 std::mutex g_graph_mu;
 std::map<GraphKey, hipGraphExec_t> g_graphs;
-std::vector<hipStream_t> g_capture_streams;
+// one private capture stream per thread and device: threads that capture at
+// the same time (a serving loop and its gradient worker) never share one
+thread_local std::vector<hipStream_t> tl_capture_streams;
 constexpr size_t kMaxGraphs = 64;
 
-// Replays (capturing on first use) the graph of one chunk.
-int run_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective,
-              void* spec_a, void* spec_b, float* u_final, double* jpart, int jcap, float* ckpt,
-              int ckpt_every, int ckpt_cap, bool use_graph, hipStream_t st) {
-  if (!use_graph) {
-    HIP_TRY(enqueue_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart, jcap,
-                          ckpt, ckpt_every, ckpt_cap, st));
-    return NSK_OK;
-  }
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  GraphKey k{dev, p->N, p->batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap,
-             slots, spec_a, spec_b, u_final, jpart, ckpt, p->L, (double)p->mu0,
-             (double)p->sigma0, (double)p->C};
+// Replays the graph of `k`, capturing `enqueue` (which only launches
+// kernels) into it on first use.
+template <class F>
+int graph_replay(const GraphKey& k, F&& enqueue, hipStream_t st) {
   hipGraphExec_t exec = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_graph_mu);
@@ -368,31 +361,56 @@ int run_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int obj
   if (!exec) {
     // Capture on a private stream (the caller's may be the NULL stream,
     // which cannot be captured); capturing records work without running it.
-    hipStream_t cs = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(g_graph_mu);
-      if ((int)g_capture_streams.size() <= dev) g_capture_streams.resize(dev + 1, nullptr);
-      if (!g_capture_streams[dev]) HIP_TRY(hipStreamCreateWithFlags(&g_capture_streams[dev], hipStreamNonBlocking));
-      cs = g_capture_streams[dev];
-    }
+    if ((int)tl_capture_streams.size() <= k.dev) tl_capture_streams.resize(k.dev + 1, nullptr);
+    if (!tl_capture_streams[k.dev])
+      HIP_TRY(hipStreamCreateWithFlags(&tl_capture_streams[k.dev], hipStreamNonBlocking));
+    hipStream_t cs = tl_capture_streams[k.dev];
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart,
-                                 jcap, ckpt, ckpt_every, ckpt_cap, cs);
+    hipError_t e = enqueue(cs);
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
-    if (e != hipSuccess) return fail(NSK_E_HIP, "capture of rollout chunk failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(NSK_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(NSK_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
     HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     HIP_TRY(hipGraphDestroy(graph));
     std::lock_guard<std::mutex> lk(g_graph_mu);
-    if (g_graphs.size() >= kMaxGraphs) {
-      for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
-      g_graphs.clear();
+    auto it = g_graphs.find(k);
+    if (it != g_graphs.end()) {
+      // another thread captured the same key meanwhile
+      (void)hipGraphExecDestroy(exec);
+      exec = it->second;
+    } else {
+      if (g_graphs.size() >= kMaxGraphs) {
+        // cached graphs may still run on some stream: let them finish first
+        HIP_TRY(hipDeviceSynchronize());
+        for (auto& kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+        g_graphs.clear();
+      }
+      g_graphs[k] = exec;
     }
-    g_graphs[k] = exec;
   }
   HIP_TRY(hipGraphLaunch(exec, st));
   return NSK_OK;
+}
+
+// Replays (capturing on first use) the graph of one chunk.
+int run_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective,
+              void* spec_a, void* spec_b, float* u_final, double* jpart, int jcap, float* ckpt,
+              int ckpt_every, int ckpt_cap, bool use_graph, hipStream_t st) {
+  auto enqueue = [&](hipStream_t s) {
+    return enqueue_chunk(p, slots, chunk, scheme, objective, spec_a, spec_b, u_final, jpart, jcap, ckpt,
+                         ckpt_every, ckpt_cap, s);
+  };
+  if (!use_graph) {
+    HIP_TRY(enqueue(st));
+    return NSK_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  const GraphKey k{0, dev, p->N, p->batch, chunk, scheme, objective, jcap, ckpt_every, ckpt_cap,
+                   slots, spec_a, spec_b, u_final, jpart, ckpt, p->L, (double)p->mu0,
+                   (double)p->sigma0, (double)p->C, 0.0};
+  return graph_replay(k, enqueue, st);
 }
 
 // Steps `steps` from the predictor spectrum in *cur; keeps the ping-pong
@@ -645,10 +663,44 @@ int nsk_rollout(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* sl
                       ckpt_every, use_graph != 0, (hipStream_t)stream);
 }
 
+// One chunk of the reverse sweep when every state is checkpointed: steps
+// n = step + o for o = chunk-1 .. 0 (projection of lambda, then the
+// transposed predictor reading u_n from ckpt), then step -= chunk.  The
+// launches depend on the slots' step only through the device, so one
+// captured chunk serves every chunk of the sweep.  Ping-pongs between
+// lam_a and lam_b.
+static hipError_t enqueue_adj_chunk(const nsk_plan* p, nsk_slot* slots, int chunk, int scheme, int objective,
+                                    float jseed, void* lam_a, void* lam_b, double* gpart, int steps,
+                                    const float* ckpt, int ncheck, hipStream_t st) {
+  const long long NN2 = 2LL * p->N * p->N;
+  void* lam = lam_a;
+  void* lam_next = lam_b;
+  for (int o = chunk - 1; o >= 0; --o) {
+    hipError_t e = launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, o, REQ_ACTIVE), p->batch, st);
+    if (e != hipSuccess) return e;
+    RowArgs a = row_args(p, slots);
+    a.src_spec = (const float4*)lam;
+    a.dst_spec = (float4*)lam_next;
+    a.step_off = o;
+    a.require = REQ_ACTIVE;
+    a.part = gpart;
+    a.pcap = steps;
+    a.objective = objective;
+    a.jseed = jseed;
+    a.ustate = ckpt;
+    a.us_bstride = (long long)ncheck * NN2;
+    a.us_base = 0;
+    e = launch_row(p->N, RK_ADJ, scheme, a, p->batch, st);
+    if (e != hipSuccess) return e;
+    std::swap(lam, lam_next);
+  }
+  return add_step(slots, p->batch, -chunk, st);
+}
+
 static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int scheme, int objective,
                         float jseed, const float* lam_final, double* dtheta_out, float* du0_out,
                         void* spec0, void* spec1, void* spec2, void* spec3, double* gpart,
-                        const float* ckpt, float* seg, int ckpt_every, hipStream_t st) {
+                        const float* ckpt, float* seg, int ckpt_every, bool use_graph, hipStream_t st) {
   const long long NN2 = 2LL * p->N * p->N;
   const int ncheck = (steps + ckpt_every - 1) / ckpt_every;
   // lambda_steps: the terminal seed (row spectrum of dL/du_final) or 0
@@ -669,24 +721,47 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
   void* lam_next = spec3;
   if (ckpt_every == 1) {
     // every state is checkpointed: the reverse sweep reads u_n straight from
-    // the checkpoints, nothing is recomputed
-    HIP_TRY(set_step(slots, p->batch, 0, st));
-    for (int n = steps - 1; n >= 0; --n) {
-      HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, n, REQ_ACTIVE), p->batch, st));
-      RowArgs a = row_args(p, slots);
-      a.src_spec = (const float4*)lam;
-      a.dst_spec = (float4*)lam_next;
-      a.step_off = n;
-      a.require = REQ_ACTIVE;
-      a.part = gpart;
-      a.pcap = steps;
-      a.objective = objective;
-      a.jseed = jseed;
-      a.ustate = ckpt;
-      a.us_bstride = (long long)ncheck * NN2;
-      a.us_base = 0;
-      HIP_TRY(launch_row(p->N, RK_ADJ, scheme, a, p->batch, st));
-      std::swap(lam, lam_next);
+    // the checkpoints, nothing is recomputed.  Full chunks of kAdjChunk
+    // steps from the top (n = steps-1 down), graph-replayed; the remaining
+    // steps n < rem last, launched directly.
+    constexpr int kAdjChunk = 32;  // even: lam/lam_next end where they began
+    const int full = steps / kAdjChunk, rem = steps % kAdjChunk;
+    if (full > 0) {
+      HIP_TRY(set_step(slots, p->batch, steps - kAdjChunk, st));
+      int dev = 0;
+      HIP_TRY(hipGetDevice(&dev));
+      const GraphKey k{1, dev, p->N, p->batch, kAdjChunk, scheme, objective, steps, 1, ncheck,
+                       slots, lam, lam_next, nullptr, gpart, ckpt, p->L, (double)p->mu0,
+                       (double)p->sigma0, (double)p->C, (double)jseed};
+      auto enqueue = [&](hipStream_t s) {
+        return enqueue_adj_chunk(p, slots, kAdjChunk, scheme, objective, jseed, lam, lam_next, gpart, steps,
+                                 ckpt, ncheck, s);
+      };
+      for (int c = 0; c < full; ++c) {
+        int rc = use_graph ? graph_replay(k, enqueue, st) : (enqueue(st) == hipSuccess ? NSK_OK : -1);
+        if (rc == -1) return fail(NSK_E_HIP, "adjoint chunk launch failed: %s", hipGetErrorString(hipGetLastError()));
+        if (rc) return rc;
+      }
+    }
+    if (rem > 0) {
+      HIP_TRY(set_step(slots, p->batch, 0, st));
+      for (int n = rem - 1; n >= 0; --n) {
+        HIP_TRY(launch_col(p->N, COL_PROJECT, col_args(p, lam, slots, n, REQ_ACTIVE), p->batch, st));
+        RowArgs a = row_args(p, slots);
+        a.src_spec = (const float4*)lam;
+        a.dst_spec = (float4*)lam_next;
+        a.step_off = n;
+        a.require = REQ_ACTIVE;
+        a.part = gpart;
+        a.pcap = steps;
+        a.objective = objective;
+        a.jseed = jseed;
+        a.ustate = ckpt;
+        a.us_bstride = (long long)ncheck * NN2;
+        a.us_base = 0;
+        HIP_TRY(launch_row(p->N, RK_ADJ, scheme, a, p->batch, st));
+        std::swap(lam, lam_next);
+      }
     }
   }
   for (int sg = (ckpt_every == 1) ? -1 : ncheck - 1; sg >= 0; --sg) {
@@ -761,7 +836,7 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
 int nsk_rollout_adjoint(const nsk_plan* p, nsk_slot* slots, int steps, int scheme, int objective,
                         float jseed, const float* lam_final, double* dtheta_out, float* du0_out,
                         void* spec0, void* spec1, void* spec2, void* spec3, double* gpart,
-                        const float* ckpt, float* seg, int ckpt_every, void* stream) {
+                        const float* ckpt, float* seg, int ckpt_every, int use_graph, void* stream) {
   CHECK_PLAN(p);
   int rc = check_common(p, steps, scheme, objective);
   if (rc) return rc;
@@ -770,13 +845,13 @@ int nsk_rollout_adjoint(const nsk_plan* p, nsk_slot* slots, int steps, int schem
   if (!slots || !spec0 || !spec1 || !spec2 || !spec3 || !gpart || !ckpt || (!seg && ckpt_every > 1))
     return fail(NSK_E_INVAL, "nsk_rollout_adjoint: NULL buffer");
   return adjoint_impl(p, slots, steps, scheme, objective, jseed, lam_final, dtheta_out, du0_out, spec0, spec1,
-                      spec2, spec3, gpart, ckpt, seg, ckpt_every, (hipStream_t)stream);
+                      spec2, spec3, gpart, ckpt, seg, ckpt_every, use_graph != 0, (hipStream_t)stream);
 }
 
 int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
                      int scheme, int objective, double* J_out, double* dtheta_out, float* du0_out,
                      void* spec0, void* spec1, void* spec2, void* spec3, double* jpart, double* gpart,
-                     float* ckpt, float* seg, int ckpt_every, void* stream) {
+                     float* ckpt, float* seg, int ckpt_every, int use_graph, void* stream) {
   CHECK_PLAN(p);
   int rc = check_common(p, steps, scheme, objective);
   if (rc) return rc;
@@ -787,10 +862,10 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
     return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
   hipStream_t st = (hipStream_t)stream;
   rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
-                    ckpt_every, false, st);
+                    ckpt_every, use_graph != 0, st);
   if (rc) return rc;
   return adjoint_impl(p, slots, steps, scheme, objective, 1.0f, nullptr, dtheta_out, du0_out, spec0, spec1,
-                      spec2, spec3, gpart, ckpt, seg, ckpt_every, st);
+                      spec2, spec3, gpart, ckpt, seg, ckpt_every, use_graph != 0, st);
 }
 
 int nsk_admit(const nsk_plan* p, const float* u0, nsk_slot* slots, int scheme, int objective, void* spec,

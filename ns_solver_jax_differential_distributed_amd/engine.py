@@ -116,10 +116,13 @@ class RolloutEngine:
         return u.contiguous()
 
     # ---------------------------------------------------------------- forward
-    def rollout(self, u0, nu, dt, theta, steps, ckpt_every: Optional[int] = None):
+    def rollout(self, u0, nu, dt, theta, steps, ckpt_every: Optional[int] = None, graph_ckpt: bool = False):
         """(J (B,) float64, u_final (B,2,N,N), ckpt or None).  ``steps`` is an
         int or per-slot (B,) counts.  With ckpt_every, u_n is stored every
-        ckpt_every steps for the adjoint."""
+        ckpt_every steps for the adjoint; the step chunks are then replayed
+        from HIP graphs only with graph_ckpt (graphs are cached per buffer
+        address: callers that keep several checkpoint sets alive would only
+        capture new ones)."""
         counts = self.step_counts(steps)
         steps = int(counts.max())
         u0 = self.field(u0)
@@ -138,17 +141,19 @@ class RolloutEngine:
             ctypes.byref(self.plan), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
             _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective], J.data_ptr(),
             self.spec(0).data_ptr(), self.spec(1).data_ptr(), jpart.data_ptr(),
-            None if ckpt is None else ckpt.data_ptr(), k, int(self.use_graph and ckpt is None),
+            None if ckpt is None else ckpt.data_ptr(), k, int(self.use_graph and (ckpt is None or graph_ckpt)),
             _lib.stream_ptr()), "rollout")
         return J, u_final, ckpt
 
     # ---------------------------------------------------------------- adjoint
     def adjoint(self, nu, dt, theta, steps, ckpt: torch.Tensor, ckpt_every: int,
-                jseed=1.0, lam_final: Optional[torch.Tensor] = None, want_du0: bool = True):
+                jseed=1.0, lam_final: Optional[torch.Tensor] = None, want_du0: bool = True,
+                use_graph: bool = False):
         """Reverse sweep from checkpoints of a previous rollout with the same
         arguments.  Returns (dJ/dtheta (B,2) float64, dJ/du0 (B,2,N,N) or
         None) for the seeds jseed = dL/dJ (scalar or (B,)) and lam_final =
-        dL/du_final (needs equal step counts)."""
+        dL/du_final (needs equal step counts).  use_graph: replay the reverse
+        sweep (every state checkpointed) from HIP graphs."""
         counts = self.step_counts(steps)
         steps = int(counts.max())
         if lam_final is not None and (counts != steps).any():
@@ -169,7 +174,8 @@ class RolloutEngine:
             _lib.OBJECTIVE[self.objective], 1.0, None if lf is None else lf.data_ptr(),
             dth.data_ptr(), None if du0 is None else du0.data_ptr(),
             *[self.spec(i).data_ptr() for i in range(4)], gpart.data_ptr(), ckpt.data_ptr(),
-            None if seg is None else seg.data_ptr(), k, _lib.stream_ptr()), "rollout_adjoint")
+            None if seg is None else seg.data_ptr(), k, int(self.use_graph and use_graph), _lib.stream_ptr()),
+            "rollout_adjoint")
         return dth, du0
 
     AUTO_CKPT_BYTES = 8 << 30
@@ -195,8 +201,11 @@ class RolloutEngine:
             z = torch.zeros((self.batch,), dtype=torch.float64, device=u0.device)
             return z, u0.clone(), torch.zeros((self.batch, 2), dtype=torch.float64,
                                               device=u0.device), torch.zeros_like(u0)
-        J, uf, ckpt = self.rollout(u0, nu, dt, theta, steps, ckpt_every)
-        dth, du0 = self.adjoint(nu, dt, theta, steps, ckpt, ckpt_every)
+        # the checkpoints live only for this call: freed and reallocated at the
+        # same size they normally come back at the same address, so the
+        # cached graphs are replayed
+        J, uf, ckpt = self.rollout(u0, nu, dt, theta, steps, ckpt_every, graph_ckpt=True)
+        dth, du0 = self.adjoint(nu, dt, theta, steps, ckpt, ckpt_every, use_graph=True)
         return J, uf, dth, du0
 
 

@@ -158,6 +158,33 @@ def test_gradient_without_recomputation_is_bitwise_equal(scheme):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
+def test_graph_replayed_gradient_equals_eager(scheme):
+    """75 steps with every state kept: the forward in graph-replayed chunks and
+    the reverse sweep as two 32-step chunks replayed from one captured graph
+    plus 11 steps launched directly.  Equal bits to the eager forward and
+    sweep and to the recomputing sweep (ckpt_every=5), on three calls in a
+    row with eager work between them."""
+    N, L, nu, dt, steps = 64, 1.0, 0.01, 1e-3, 75
+    rng = np.random.default_rng(11)
+    u0 = torch.from_numpy((0.4 * rng.standard_normal((2, N, N))).astype(np.float32))
+    theta = [0.9, 1.1]
+    ref = [t.cpu() for t in solver(N, L, nu, dt, steps, scheme, ckpt_every=5,
+                                   use_graph=False).rollout_and_grad(u0, theta)]
+    eager = [t.cpu() for t in solver(N, L, nu, dt, steps, scheme, ckpt_every=1,
+                                     use_graph=False).rollout_and_grad(u0, theta)]
+    for a, b in zip(ref, eager):
+        assert torch.equal(a, b)
+    g = solver(N, L, nu, dt, steps, scheme, ckpt_every=1)
+    other = solver(N, L, nu, dt, 9, scheme, use_graph=False)
+    for _ in range(3):
+        out = [t.cpu() for t in g.rollout_and_grad(u0, theta)]
+        for a, b in zip(ref, out):
+            assert torch.equal(a, b)
+        other.rollout(u0, theta)
+        torch.cuda.synchronize()
+
+
 def test_jit_gradient_matches_jax_reverse_mode():
     """JAX's gradient at theta=[1,0.5] from the reference's SGD(lr=0.1)
     trajectory (studies.py:890-918): g = (theta_0 - theta_1)/0.1;
