@@ -34,6 +34,9 @@
 
 // Keeps the scheduler from interleaving independent lines (and so from
 // holding the working sets of several transforms in registers at once).
+#ifndef NSK_XREAD_ALL
+#define NSK_XREAD_ALL 0
+#endif
 #ifndef NSK_NO_SCHED_FENCE
 #define NSK_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
 #else
@@ -272,8 +275,17 @@ __device__ __forceinline__ void xwrite(const float2 (&v)[fft_e(N, E)], float2* L
     const int b = t + T * q;
     const int k = b & (P - 1);
     const int base = (b - k) * R + k;
+    if constexpr (P % 16 == 0 || (P == 1 && R == 16)) {
+      // pad(base + j P) = pad(base) + pad(j P) (P a multiple of 16), or
+      // pad(base) + j (base a multiple of 16, j < 16): one address per q,
+      // immediate offsets per j
+      float2* Lb = L + pad(base);
 #pragma unroll
-    for (int j = 0; j < R; ++j) L[pad(base + j * P)] = v[q + j * Q];
+      for (int j = 0; j < R; ++j) Lb[P % 16 == 0 ? pad(j * P) : j] = v[q + j * Q];
+    } else {
+#pragma unroll
+      for (int j = 0; j < R; ++j) L[pad(base + j * P)] = v[q + j * Q];
+    }
   }
 }
 
@@ -281,8 +293,15 @@ __device__ __forceinline__ void xwrite(const float2 (&v)[fft_e(N, E)], float2* L
 template <int N, int E>
 __device__ __forceinline__ void xread(float2 (&v)[fft_e(N, E)], const float2* L, int t) {
   constexpr int EE = RF<N, E>::EE, T = RF<N, E>::T;
+  if constexpr (T % 16 == 0) {
+    // pad(t + T m) = pad(t) + pad(T m): one address, immediate offsets
+    const float2* Lt = L + pad(t);
 #pragma unroll
-  for (int m = 0; m < EE; ++m) v[m] = L[pad(t + T * m)];
+    for (int m = 0; m < EE; ++m) v[m] = Lt[pad(T * m)];
+  } else {
+#pragma unroll
+    for (int m = 0; m < EE; ++m) v[m] = L[pad(t + T * m)];
+  }
 }
 
 // LDS exchange buffers of one thread: line l of buffer k (k = 0, 1) sits
@@ -320,12 +339,24 @@ __device__ __forceinline__ void rfft_from(float2 (*v)[fft_e(N, E)], XBuf<DB>& xb
 #pragma unroll
     for (int l = 0; l < NL; ++l) xwrite<N, E, s - 1>(v[l], B + l * xb.lstride, t);
     __syncthreads();
+#if NSK_XREAD_ALL
+    // every line's reads in flight before the first line's pass: the later
+    // lines' LDS latency hides behind the earlier lines' arithmetic
+#pragma unroll
+    for (int l = 0; l < NL; ++l) xread<N, E>(v[l], B + l * xb.lstride, t);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      rpass<N, E, s, DIR>(v[l], tw, t);
+      NSK_SCHED_FENCE();
+    }
+#else
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       xread<N, E>(v[l], B + l * xb.lstride, t);
       rpass<N, E, s, DIR>(v[l], tw, t);
       NSK_SCHED_FENCE();
     }
+#endif
     rfft_from<N, E, s + 1, DIR, NL, DB>(v, xb, t, tw);
   }
 }

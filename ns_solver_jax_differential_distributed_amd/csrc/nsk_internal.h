@@ -83,9 +83,12 @@ struct StencilArgs {
 #endif
 template <int N>
 struct RowGeo;
-template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = 8, RMAX = 16; static constexpr bool DB = true; };
-template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
-template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
+#ifndef NSK_ROW_DB
+#define NSK_ROW_DB 1
+#endif
+template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = 8, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
+template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
+template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
 template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
 template <> struct RowGeo<256> { static constexpr int E = 16, LG = 4, RMIN = 2, RMAX = 8; static constexpr bool DB = true; };
 template <> struct RowGeo<128> { static constexpr int E = 16, LG = 8, RMIN = 2, RMAX = 4; static constexpr bool DB = true; };

@@ -231,7 +231,10 @@ struct RowK {
   static constexpr int T = LG * TL;
   static constexpr int NW = TL >= 64 ? TL / 64 : 0;  // waves per row stream
   static constexpr int LS = line_stride(LineLen<N>::NP, TL < 16 ? TL : 0);
-  static constexpr int WPE = (EE == 16) ? 2 : 4;  // waves per SIMD aimed at (register budget)
+#ifndef NSK_ROW_WPE16
+#define NSK_ROW_WPE16 2
+#endif
+  static constexpr int WPE = (EE == 16) ? NSK_ROW_WPE16 : 4;  // waves per SIMD aimed at (register budget)
 };
 
 // The row pass.  Workgroup `band` runs LG row streams; stream g walks the

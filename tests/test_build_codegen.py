@@ -4,7 +4,8 @@
 * A 12- or 16-byte VMEM store whose data registers are overwritten by the
   very next (VALU) instruction: with an SGPR offset the compiler treats
   that sequence as hazard-free, and on gfx950 it corrupted stored lanes.
-* The hot kernels (fused row pass, column projection) must not spill.
+* The hot kernels (fused row pass, reverse sweep, column projection) must
+  not spill.
 """
 import os
 import re
@@ -72,6 +73,8 @@ def test_no_wide_store_data_overwritten_by_next_instruction(asm_4096):
 @pytest.mark.parametrize("kernel", [
     "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi4ELi0EEEvNS_7RowArgsE",  # fused row pass, central
     "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi4ELi1EEEvNS_7RowArgsE",  # fused row pass, upwind
+    "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi5ELi0EEEvNS_7RowArgsE",  # reverse sweep, central
+    "_ZN3nsk12_GLOBAL__N_15k_rowILi4096ELi5ELi1EEEvNS_7RowArgsE",  # reverse sweep, upwind
     "_ZN3nsk12_GLOBAL__N_15k_colILi4096ELi0EEEvNS_7ColArgsE",      # column projection
 ])
 def test_hot_kernels_do_not_spill(asm_4096, kernel):
