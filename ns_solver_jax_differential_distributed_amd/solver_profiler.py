@@ -1,0 +1,174 @@
+"""Stage-wise timing of the solver (the reference's ``solver_profiler.py``).
+
+``ProfileSummary`` and ``SolverKernelProfiler.benchmark_callable`` keep the
+reference's contract (solver_profiler.py:14-61): ``warmup`` untimed calls,
+then ``runs`` timed calls, each waited for before the clock stops (here
+``torch.cuda.synchronize()`` plays the part of ``jax.block_until_ready``);
+mean / min / max / std of the wall times in seconds.
+
+``profile_solver_kernels`` times the reference's single-core stages
+(solver_profiler.py:92-142) on this build's operator-level kernels - forcing,
+advection-diffusion predictor, pressure solve, projection, one step, the
+rollout - and adds the two fused passes a rollout step actually runs, timed
+with HIP events on the launch stream:
+
+  single: row_pass (fused)     inverse row FFT, predictor, forward row FFT
+  single: column_pass (fused)  column FFT, pressure projection, inverse
+
+The distributed rows of the reference need several GPUs and are skipped
+with a log line (the ``distributed`` argument is accepted for signature
+compatibility and must be None).
+"""
+from __future__ import annotations
+
+import ctypes
+import logging
+import time
+from dataclasses import dataclass
+from typing import Any, Callable, List, Optional
+
+import numpy as np
+import torch
+
+
+@dataclass(frozen=True)
+class ProfileSummary:
+    label: str
+    mean_sec: float
+    min_sec: float
+    max_sec: float
+    std_sec: float
+
+
+def _summary(label: str, samples: List[float]) -> ProfileSummary:
+    arr = np.array(samples, dtype=np.float64)
+    return ProfileSummary(label=label, mean_sec=float(arr.mean()), min_sec=float(arr.min()),
+                          max_sec=float(arr.max()), std_sec=float(arr.std()))
+
+
+class SolverKernelProfiler:
+    """Times solver stages (solver_profiler.py:23-61, 72-236)."""
+
+    def __init__(self, mesh: Any = None, logger: Optional[logging.Logger] = None):
+        self.mesh = mesh
+        self.logger = logger if logger is not None else logging.getLogger(__name__)
+
+    @staticmethod
+    def _block(x: Any) -> Any:
+        """Wait for the GPU work behind ``x`` (all of the device's streams)."""
+        if torch.cuda.is_available() and torch.cuda.is_initialized():
+            torch.cuda.synchronize()
+        return x
+
+    @classmethod
+    def benchmark_callable(cls, label: str, fn: Callable, *args, warmup: int = 1,
+                           runs: int = 5) -> ProfileSummary:
+        """Wall time of ``fn(*args)`` over ``runs`` calls after ``warmup``
+        untimed ones."""
+        for _ in range(max(0, warmup)):
+            cls._block(fn(*args))
+        samples: List[float] = []
+        for _ in range(max(1, runs)):
+            t0 = time.perf_counter()
+            cls._block(fn(*args))
+            samples.append(time.perf_counter() - t0)
+        return _summary(label, samples)
+
+    @staticmethod
+    def benchmark_device(label: str, fn: Callable, *args, warmup: int = 1, runs: int = 5) -> ProfileSummary:
+        """Device time of ``fn(*args)`` (work it enqueues on the current
+        stream), from HIP events around each call."""
+        for _ in range(max(0, warmup)):
+            fn(*args)
+        samples = []
+        for _ in range(max(1, runs)):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn(*args)
+            e1.record()
+            e1.synchronize()
+            samples.append(e0.elapsed_time(e1) * 1e-3)
+        return _summary(label, samples)
+
+    def _log_profile_table(self, title: str, rows: List[ProfileSummary]) -> None:
+        self.logger.info("-" * 80)
+        self.logger.info(title)
+        self.logger.info("-" * 80)
+        self.logger.info(f"{'Kernel':36s} {'mean [s]':>12s} {'min [s]':>12s} {'max [s]':>12s} {'std [s]':>12s}")
+        for row in rows:
+            self.logger.info(f"{row.label:36s} {row.mean_sec:12.6f} {row.min_sec:12.6f} "
+                             f"{row.max_sec:12.6f} {row.std_sec:12.6f}")
+
+    def profile_solver_kernels(self, single, distributed=None, theta=None, warmup: int = 1,
+                               runs: int = 5) -> List[ProfileSummary]:
+        """Stage rows for the single-core solver (the reference's labels),
+        plus the fused passes; returns and logs them."""
+        from . import _lib
+        if distributed is not None:
+            raise ValueError("distributed profiling needs several GPUs; pass distributed=None")
+        N, dt = single.grid.N, single.time.dt
+        theta = np.asarray([1.0, 0.5] if theta is None else theta, dtype=np.float32)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        u0 = torch.zeros((2, N, N), dtype=torch.float32, device=dev)
+        ops, adv, poi = single.projector.ops, single.projector.advection, single.projector.poisson
+        nu = single.projector.nu
+
+        def forcing(th, t):
+            return single.forcing(th, t)
+
+        def advec_diff(u, f):
+            ux, uy = u[0], u[1]
+            lap_ux, lap_uy = ops.laplacian(ux), ops.laplacian(uy)
+            adv_ux, adv_uy = adv.advect(ux, ux, uy, ops), adv.advect(uy, ux, uy, ops)
+            return torch.stack([ux + dt * (nu * lap_ux + adv_ux + f[0]),
+                                uy + dt * (nu * lap_uy + adv_uy + f[1])])
+
+        def pressure(u_star):
+            return poi(ops.divergence(u_star) / dt)
+
+        def projection(u_star, p):
+            px, py = ops.grad(p)
+            return torch.stack([u_star[0] - dt * px, u_star[1] - dt * py])
+
+        f0 = forcing(theta, 0.0)
+        us = advec_diff(u0, f0)
+        p0 = pressure(us)
+        rows = [
+            self.benchmark_callable("single: forcing", forcing, theta, 0.0, warmup=warmup, runs=runs),
+            self.benchmark_callable("single: advection_diffusion", advec_diff, u0, f0, warmup=warmup, runs=runs),
+            self.benchmark_callable("single: pressure_solve", pressure, us, warmup=warmup, runs=runs),
+            self.benchmark_callable("single: projection", projection, us, p0, warmup=warmup, runs=runs),
+            self.benchmark_callable("single: full_step", single.step, u0, theta, 0.0, warmup=warmup, runs=runs),
+            self.benchmark_callable("single: full_rollout", single.rollout, u0, theta, warmup=warmup,
+                                    runs=max(1, min(runs, 3))),
+        ]
+        # the two passes of a rollout step, on the solver's own buffers
+        eng = single.engine
+        eng.rollout(u0, single.phys.nu, dt, theta, 2)
+        lib, plan = _lib.lib(), eng.plan
+        s0, s1 = eng.spec(0), eng.spec(1)
+        slots = _lib.make_slots(single.phys.nu, dt, float(theta[0]), float(theta[1]), 0, 1 << 30)
+        jpart = eng.buf("prof_jpart", (eng.bands(),), torch.float64)
+
+        def row_pass():
+            _lib.check(lib.nsk_row_pass(ctypes.byref(plan), s0.data_ptr(), s1.data_ptr(), slots.data_ptr(), 0,
+                                        _lib.SCHEME[single.scheme], _lib.OBJECTIVE[single.objective_kind],
+                                        jpart.data_ptr(), 1, _lib.stream_ptr()), "row_pass")
+
+        def column_pass():
+            _lib.check(lib.nsk_column_pass(ctypes.byref(plan), s0.data_ptr(), 0, _lib.stream_ptr()),
+                       "column_pass")
+
+        rows += [self.benchmark_device("single: row_pass (fused)", row_pass, warmup=warmup, runs=runs),
+                 self.benchmark_device("single: column_pass (fused)", column_pass, warmup=warmup, runs=runs)]
+        self._log_profile_table("PROFILE SUMMARY (single-core)", rows)
+        self.logger.info("Distributed stages skipped: one GPU.")
+        return rows
+
+
+def profile_solver_kernels(single, distributed=None, mesh: Any = None, theta=None, warmup: int = 1,
+                           runs: int = 5, logger: Optional[logging.Logger] = None) -> List[ProfileSummary]:
+    """Function wrapper around SolverKernelProfiler (solver_profiler.py:239-251)."""
+    return SolverKernelProfiler(mesh=mesh, logger=logger).profile_solver_kernels(
+        single, distributed, theta, warmup=warmup, runs=runs)
