@@ -114,7 +114,8 @@ class _Bucket:
         self.nb = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(self.plan1), 1))
         self.jcap = srv.max_steps
         dev = torch.device("cuda", torch.cuda.current_device())
-        self.stream = torch.cuda.Stream()
+        self.stream = srv.stream_for(N)
+        self.chunk = srv.chunk_for(N, capacity)
         with torch.cuda.stream(self.stream):
             self.spec_a = torch.zeros((capacity, N, N // 2, 4), dtype=torch.float32, device=dev)
             self.spec_b = torch.zeros_like(self.spec_a)
@@ -172,7 +173,7 @@ class _Bucket:
     def advance(self) -> List[int]:
         """One chunk for every slot; returns the slots that finished in it."""
         lib = _lib.lib()
-        k = self.srv.chunk
+        k = self.chunk
         with torch.cuda.stream(self.stream):
             _lib.check(lib.nsk_advance(ctypes.byref(self.plan), self.slots.data_ptr(), k,
                                        _lib.SCHEME[self.srv.scheme], _lib.OBJECTIVE[self.srv.objective],
@@ -231,7 +232,7 @@ class _GradLane:
 
     def __init__(self, srv: "NSServer", N: int):
         self.srv, self.N = srv, N
-        self.stream = torch.cuda.Stream()
+        self.stream = srv.stream_for(N, grad=True)
         self.engines: Dict[int, RolloutEngine] = {}
         self.running = None      # (future -> (event, outputs), reqs, bytes, admitted)
 
@@ -296,21 +297,41 @@ class NSServer:
 
     L, phys: domain length and (mu0, sigma0, C) shared by all requests;
     capacity: slots per forward bucket, per N; chunk: steps per bucket
-    launch (even); hbm_budget: bytes the server may allocate."""
+    launch (even), or None to size each bucket's chunk to about chunk_us of
+    GPU time (estimated from N and the bucket's capacity); streams:
+    "shared" (one stream for the buckets up to N = SMALL_N, one for the
+    larger ones, one for gradient groups) or "per_bucket"; hbm_budget:
+    bytes the server may allocate.
+
+    The device runs a few hardware queues, and streams beyond their number
+    share them in order: a stream of small kernels placed behind a long
+    sequence (a large bucket's chunk, a gradient group) waits for all of it.
+    Shared streams keep the small buckets on a queue of their own, and
+    chunks sized by GPU time keep any one launch short."""
 
     DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 48, 1024: 16, 2048: 6, 4096: 2}
+    SMALL_N = 256
+    # steady cell-updates/s of a full launch and the floor of one step's two
+    # launches, for sizing chunks (measured on MI355X)
+    CELL_RATE = 1.1e11
+    STEP_FLOOR_US = 20.0
 
     def __init__(self, L: float = 2 * math.pi, phys=(0.5, 0.1, 1.0), scheme: str = "central",
-                 objective: str = "work", chunk: int = 16, max_steps: int = 512,
+                 objective: str = "work", chunk: Optional[int] = 16, max_steps: int = 512,
                  capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 96 << 30,
-                 grad_group: int = 64, grad_ckpt_every: int = 25, use_graph: bool = True):
-        if chunk < 2 or chunk % 2:
+                 grad_group: int = 64, grad_ckpt_every: int = 25, use_graph: bool = True,
+                 streams: str = "per_bucket", chunk_us: float = 500.0):
+        if chunk is not None and (chunk < 2 or chunk % 2):
             raise ValueError("chunk must be an even number >= 2")
+        if streams not in ("shared", "per_bucket"):
+            raise ValueError("streams must be 'shared' or 'per_bucket'")
         if scheme not in _lib.SCHEME or objective not in _lib.OBJECTIVE:
             raise ValueError("bad scheme or objective")
         self.L, self.phys = float(L), tuple(float(v) for v in phys)
         self.scheme, self.objective = scheme, objective
-        self.chunk, self.max_steps = int(chunk), int(max_steps)
+        self.chunk, self.max_steps = (None if chunk is None else int(chunk)), int(max_steps)
+        self.streams, self.chunk_us = streams, float(chunk_us)
+        self._shared: Dict[str, torch.cuda.Stream] = {}
         self.capacity = dict(self.DEFAULT_CAPACITY if capacity is None else capacity)
         self.budget = HBMBudget(hbm_budget)
         self.grad_group, self.grad_ckpt_every = int(grad_group), int(grad_ckpt_every)
@@ -326,6 +347,25 @@ class NSServer:
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
+
+    def stream_for(self, N: int, grad: bool = False) -> torch.cuda.Stream:
+        """The stream of N's bucket (grad=False) or gradient lane."""
+        if self.streams == "per_bucket":
+            return torch.cuda.Stream()
+        key = "grad" if grad else ("small" if N <= self.SMALL_N else "large")
+        st = self._shared.get(key)
+        if st is None:
+            st = self._shared[key] = torch.cuda.Stream()
+        return st
+
+    def chunk_for(self, N: int, capacity: int) -> int:
+        """Steps per launch of a bucket: the fixed ``chunk``, or an even count
+        of about chunk_us of estimated GPU time (2..64)."""
+        if self.chunk is not None:
+            return self.chunk
+        step_us = max(self.STEP_FLOOR_US, 1e6 * capacity * N * N / self.CELL_RATE)
+        k = int(round(self.chunk_us / step_us / 2)) * 2
+        return max(2, min(64, k))
 
     # ---------------------------------------------------------------- requests
     def submit(self, req: SolveRequest) -> int:

@@ -56,6 +56,33 @@ def test_served_request_equals_plain_solver_and_is_mix_independent():
         assert torch.equal(res[r.rid].u_final, ur) and res[r.rid].J == float(Jr)
 
 
+def test_shared_streams_and_timed_chunks_give_the_same_results():
+    """streams="shared" (small buckets, large buckets and gradient groups on
+    three streams) with chunks sized by GPU time: a mix of forward and
+    gradient requests equals the plain per-request results bit for bit."""
+    pkg, serving, engine = M()
+    L = 2 * math.pi
+    reqs = [_req(serving, N, st, seed=40 + i, grad=g) for i, (N, st, g) in
+            enumerate([(32, 20, False), (64, 50, True), (256, 33, False), (64, 12, False),
+                       (512, 26, False), (256, 40, True), (32, 61, True)])]
+    for i, r in enumerate(reqs):
+        r.arrival = 0.001 * i
+    srv = serving.NSServer(chunk=None, streams="shared", max_steps=64,
+                           capacity={32: 2, 64: 2, 256: 2, 512: 1})
+    assert srv.chunk_for(512, 1) >= 2 and srv.chunk_for(32, 2) <= 64
+    res = {r.rid: r for r in srv.run(reqs, realtime=True)}
+    assert len(res) == len(reqs)
+    for r in reqs:
+        one = pkg.SingleCoreNSSolver(pkg.make_grid(r.N, L), pkg.Phys(r.nu, 0.5, 0.1, 1.0),
+                                     pkg.Time(r.dt, r.steps))
+        if r.want_grad:
+            Jr, ur, g, _ = one.rollout_and_grad(r.u0, r.theta)
+            assert np.array_equal(res[r.rid].dtheta, g[:2].cpu().numpy())
+        else:
+            Jr, ur = one.rollout(r.u0, r.theta)
+        assert torch.equal(res[r.rid].u_final, ur) and res[r.rid].J == float(Jr)
+
+
 def test_served_gradients_equal_single_request_gradients():
     pkg, serving, engine = M()
     L = 2 * math.pi
