@@ -83,7 +83,8 @@ const char *nsk_last_error(void);
 int nsk_init(void);
 
 /* Fills *p (host memory).  Replaces the constructor arguments of
- * SingleCoreNSSolver / Grid / Phys (NS_solver.py:715-723, 112-117). */
+ * SingleCoreNSSolver / Grid / Phys (NS_solver.py:715-723, 112-117).
+ * N: a power of two in 8..4096; batch: 1..65535 slots. */
 int nsk_plan_init(nsk_plan *p, int N, double L, double mu0, double sigma0,
                   double C, int batch);
 

@@ -40,6 +40,9 @@ int fail(int code, const char* fmt, ...) {
 
 bool supported_n(int N) { return N >= 8 && N <= 4096 && (N & (N - 1)) == 0; }
 
+// slots map to the grid's y dimension
+constexpr int kMaxBatch = 65535;
+
 hipError_t launch_row(int N, int kind, int scheme, const RowArgs& a, int batch, hipStream_t s) {
   switch (N) {
 #define NSK_CASE(n) \
@@ -93,7 +96,8 @@ int check_plan(const nsk_plan* p) {
   if (!p) return fail(NSK_E_INVAL, "plan is NULL");
   if (!supported_n(p->N))
     return fail(NSK_E_UNSUPPORTED_N, "N=%d: the HIP path supports powers of two 8..4096", p->N);
-  if (p->batch < 1) return fail(NSK_E_INVAL, "batch must be >= 1, got %d", p->batch);
+  if (p->batch < 1 || p->batch > kMaxBatch)
+    return fail(NSK_E_INVAL, "batch must be in [1, %d], got %d", kMaxBatch, p->batch);
   return NSK_OK;
 }
 
@@ -449,7 +453,8 @@ int nsk_plan_init(nsk_plan* p, int N, double L, double mu0, double sigma0, doubl
   if (!supported_n(N))
     return fail(NSK_E_UNSUPPORTED_N, "N=%d: the HIP path supports powers of two 8..4096", N);
   if (!(L > 0.0)) return fail(NSK_E_INVAL, "L must be positive, got %g", L);
-  if (batch < 1) return fail(NSK_E_INVAL, "batch must be >= 1, got %d", batch);
+  if (batch < 1 || batch > kMaxBatch)
+    return fail(NSK_E_INVAL, "batch must be in [1, %d], got %d", kMaxBatch, batch);
   std::memset(p, 0, sizeof *p);
   p->N = N;
   p->batch = batch;

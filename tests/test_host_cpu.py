@@ -140,6 +140,11 @@ def test_plan_init_host_only():
             _lib.make_plan(bad, 1.0)
     with pytest.raises(ValueError):
         _lib.make_plan(64, -1.0)
+    # slots map to the launch grid's y dimension (at most 65535)
+    assert _lib.make_plan(64, 1.0, batch=65535).batch == 65535
+    for bad in (0, 65536):
+        with pytest.raises(ValueError):
+            _lib.make_plan(64, 1.0, batch=bad)
 
 
 def test_slot_record_layout():
