@@ -110,6 +110,30 @@ def test_gpu_calls_fail_loudly_without_gpu():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError):
         S.PeriodicDifferentialOps(0.1).laplacian(np.zeros((16, 16), np.float32))
+    solver = S.SingleCoreNSSolver(S.make_grid(16, 1.0), S.Phys(0.01, 0.5, 0.1, 1.0), S.Time(1e-3, 4))
+    for call in (lambda: solver.rollout(np.zeros((2, 16, 16), np.float32), [1.0, 0.5]),
+                 lambda: solver.jit_gradient([1.0, 0.5]),
+                 lambda: solver.jit_objective.batch([[1.0, 0.5], [0.5, 0.5]]),
+                 lambda: S.BatchedNSSolver(16, 1.0, 2).rollout(np.zeros((2, 2, 16, 16), np.float32),
+                                                               [1.0, 0.5], 0.01, 1e-3, 3)):
+        with pytest.raises(RuntimeError):
+            call()
+    from ns_solver_jax_differential_distributed_amd import serving
+    with pytest.raises(RuntimeError):
+        serving.NSServer().run([serving.SolveRequest(N=16, steps=2, theta=[1.0, 0.5], nu=0.01, dt=1e-3)],
+                               realtime=False)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No fallback: with the library absent every compute path raises."""
+    import subprocess
+    import sys
+    code = ("import ns_solver_jax_differential_distributed_amd._lib as L\n"
+            "try:\n    L.make_plan(64, 1.0)\nexcept RuntimeError as e:\n    print('raised', e)\n")
+    env = dict(os.environ, NSK_LIB=str(tmp_path / "absent.so"))
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("raised"), r.stdout + r.stderr
 
 
 # ---------------------------------------------------------------- the C ABI library
