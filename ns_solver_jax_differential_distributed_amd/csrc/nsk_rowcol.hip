@@ -41,6 +41,19 @@
 #ifndef NSK_COL_RCP
 #define NSK_COL_RCP 1
 #endif
+// Cache policy of the checkpoint stores (forward) and of the checkpoint
+// loads of the reverse sweep: 0 default, 2 non-temporal.  A checkpoint is
+// written once and read a whole trajectory later; stored normally it evicts
+// the spectrum the next pass reads from the last-level cache (N=4096: the
+// forward with checkpoints 240 -> 195 us per step with non-temporal stores).
+// Non-temporal loads help at N=4096 (reverse sweep 263 -> 250 us per step)
+// but not at N=1024, where neighbouring rows reuse them through L2.
+#ifndef NSK_CKPT_ST_AUX
+#define NSK_CKPT_ST_AUX 2
+#endif
+#ifndef NSK_CKPT_LD_AUX
+#define NSK_CKPT_LD_AUX (NSK_N >= 4096 ? 2 : 0)
+#endif
 
 #define NSK_CAT2(a, b) a##b
 #define NSK_CAT(a, b) NSK_CAT2(a, b)
@@ -196,6 +209,16 @@ struct Buf {
   }
   __device__ __forceinline__ void st1(float v, unsigned voff, unsigned soff) const {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  }
+  // cache policy `aux` (bit 1: non-temporal) for data written once and read
+  // much later (checkpoints), so it does not displace the working set
+  template <int AUX>
+  __device__ __forceinline__ void st1p(float v, unsigned voff, unsigned soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, AUX);
+  }
+  template <int AUX>
+  __device__ __forceinline__ float ld1p(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
   }
   __device__ __forceinline__ void st2(float2 v, unsigned voff, unsigned soff) const {
     using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
@@ -514,8 +537,8 @@ __global__ void __launch_bounds__(RowK<N>::T)
         const unsigned st_row = (unsigned)x * (4u * N) + 4u * t;
 #pragma unroll
         for (int m = 0; m < EE; ++m) {
-          bst.st1(rc[m].x, st_row, 4u * TL * m);
-          bst.st1(rc[m].y, st_row, 4u * TL * m + 4u * (unsigned)NN);
+          bst.st1p<NSK_CKPT_ST_AUX>(rc[m].x, st_row, 4u * TL * m);
+          bst.st1p<NSK_CKPT_ST_AUX>(rc[m].y, st_row, 4u * TL * m + 4u * (unsigned)NN);
         }
       }
       const float fx0 = s_f[r];
@@ -561,11 +584,12 @@ __global__ void __launch_bounds__(RowK<N>::T)
       for (int m = 0; m < EE; ++m) {
         const unsigned y = (unsigned)(t + TL * m);
         const unsigned oy = 4u * y, oym = 4u * ((y - 1u) & (N - 1)), oyp = 4u * ((y + 1u) & (N - 1));
-        const float2 Uc = make_float2(bus.ld1(ox + oy, 0), bus.ld1(ox + oy, c1));
-        const float2 Ul = make_float2(bus.ld1(oxm + oy, 0), bus.ld1(oxm + oy, c1));
-        const float2 Ur = make_float2(bus.ld1(oxp + oy, 0), bus.ld1(oxp + oy, c1));
-        const float2 Ud = make_float2(bus.ld1(ox + oym, 0), bus.ld1(ox + oym, c1));
-        const float2 Uu = make_float2(bus.ld1(ox + oyp, 0), bus.ld1(ox + oyp, c1));
+        constexpr int LA = NSK_CKPT_LD_AUX;
+        const float2 Uc = make_float2(bus.ld1p<LA>(ox + oy, 0), bus.ld1p<LA>(ox + oy, c1));
+        const float2 Ul = make_float2(bus.ld1p<LA>(oxm + oy, 0), bus.ld1p<LA>(oxm + oy, c1));
+        const float2 Ur = make_float2(bus.ld1p<LA>(oxp + oy, 0), bus.ld1p<LA>(oxp + oy, c1));
+        const float2 Ud = make_float2(bus.ld1p<LA>(ox + oym, 0), bus.ld1p<LA>(ox + oym, c1));
+        const float2 Uu = make_float2(bus.ld1p<LA>(ox + oyp, 0), bus.ld1p<LA>(ox + oyp, c1));
         const float2 c = rc[m];
         float2 dn, up;
         nbr(rc, m, x, dn, up);
