@@ -48,6 +48,13 @@
 // forward with checkpoints 240 -> 195 us per step with non-temporal stores).
 // Non-temporal loads help at N=4096 (reverse sweep 263 -> 250 us per step)
 // but not at N=1024, where neighbouring rows reuse them through L2.
+// cache policy of the row pass's spectrum loads.  Its input buffer is dead
+// once read (the next row pass overwrites it), but non-temporal loads are
+// slower (N=4096 row pass 83 -> 94 us): neighbouring bands re-read each
+// other's halo rows, which then leave the cache first.
+#ifndef NSK_ROW_IN_AUX
+#define NSK_ROW_IN_AUX 0
+#endif
 #ifndef NSK_CKPT_ST_AUX
 #define NSK_CKPT_ST_AUX 2
 #endif
@@ -217,6 +224,10 @@ struct Buf {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, AUX);
   }
   template <int AUX>
+  __device__ __forceinline__ float2 ld2p(unsigned voff, unsigned soff) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+  }
+  template <int AUX>
   __device__ __forceinline__ float ld1p(unsigned voff, unsigned soff) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
   }
@@ -353,7 +364,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
   auto load_spec = [&](float2 (&v)[EE], int x) {
     const unsigned row = (unsigned)(x & (N - 1)) * (8u * N);
 #pragma unroll
-    for (int m = 0; m < EE; ++m) v[m] = bz_in.ld2(zoff(m, row), zsoff(m));
+    for (int m = 0; m < EE; ++m) v[m] = bz_in.ld2p<NSK_ROW_IN_AUX>(zoff(m, row), zsoff(m));
   };
   auto store_spec = [&](const float2 (&v)[EE], int x) {
     const unsigned row = (unsigned)x * (8u * N);
