@@ -19,6 +19,8 @@ Also reported (one JSON line):
                 number of steps of the same workload
   accuracy      the GPU result against the FP64 oracle over a few steps of
                 the same workload (relative L2 of u, J error, divergence)
+  sweep         cell-updates/s over N in {1024, 2048, 4096} x {central,
+                upwind} x nu in {1e-4, 1e-3}
   batched, gradient, serving
                 the batched-rollout, rollout+adjoint and continuous-batching
                 configurations (see the leg_* docstrings)
@@ -168,6 +170,36 @@ def _timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def leg_sweep(steps=200, warmup=40):
+    """The throughput grid of the measurement configs: N in {1024, 2048,
+    4096} x {central, upwind} x nu in {1e-4, 1e-3}, one field each, Taylor-
+    Green + perturbation u0, dt from stable_dt for the run; cell-updates/s
+    over `steps` graph-replayed steps and the two passes' launch times."""
+    import numpy as np
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    out = []
+    for N in (1024, 2048, 4096):
+        u0 = W.initial_velocity(N, seed=0, kmax=8)
+        u0_dev = torch.from_numpy(u0).cuda()
+        theta = W.theta_batch(1, seed=0)[0]
+        for scheme in ("central", "upwind"):
+            for nu in (1e-4, 1e-3):
+                dt = W.cfl_dt(u0, L, 0.3, nu, stable=True, steps=warmup + steps)
+                s = S.SingleCoreNSSolver(S.make_grid(N, L), S.Phys(nu, 0.5, 0.1, 1.0), S.Time(dt, steps),
+                                         advection_scheme=scheme)
+                s.rollout(u0_dev, theta, steps=warmup)
+                el, (J, uf) = _timed(lambda: s.rollout(u0_dev, theta, steps=steps))
+                pt = pass_times(s, 20)
+                out.append({"N": N, "scheme": scheme, "nu": nu, "dt": dt,
+                            "cell_updates_per_s": N * N * steps / el, "us_per_step": el / steps * 1e6,
+                            "column_us": round(pt["column"] * 1e3, 2), "row_us": round(pt["row"] * 1e3, 2),
+                            "finite": bool(torch.isfinite(uf).all().item())})
+    return {"workload": f"single field, {steps} timed steps after {warmup}", "runs": out}
 
 
 def leg_batched(traj=256, N=256, steps=500):
@@ -348,7 +380,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(u0, theta, N, L, nu, dt, args.scheme, args.cpu_seconds)
         line["accuracy"] = accuracy(solver, u0, u0_dev, theta, N, L, nu, dt, args.scheme, args.accuracy_steps)
     if not args.no_extras:
-        for key, fn in (("batched", leg_batched), ("gradient", leg_gradient), ("serving", leg_serving)):
+        for key, fn in (("sweep", leg_sweep), ("batched", leg_batched), ("gradient", leg_gradient),
+                        ("serving", leg_serving)):
             try:
                 line[key] = fn()
             except Exception as e:  # a failing extra leg must not hide the main line

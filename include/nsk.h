@@ -144,7 +144,10 @@ int nsk_row_pass(const nsk_plan *p, const void *spec_in, void *spec_out,
  * slots[b].nsteps <= steps steps (its final state is written when it gets
  * there); slots[b].step must be 0 on entry and is advanced by `steps`.  J_out[b] (double, device)
  * receives the objective.  spec0/spec1: two spectrum buffers.
- * jpart: nsk_partials_per_slot(p, steps) doubles per slot.
+ * jpart: nsk_partials_per_slot(p, jcap) doubles per slot, jcap >= steps (0
+ * means steps); the cached step graphs depend on jcap and the buffer
+ * addresses, not on `steps`, so callers that keep one jpart of a fixed jcap
+ * replay the same graphs for rollouts of any length up to it.
  * ckpt (nullable): stores u_n for n % ckpt_every == 0, n < steps, as
  * ceil(steps/ckpt_every) fields of (2,N,N) per slot.
  * use_graph != 0 captures the step sequence in a HIP graph (replayed in
@@ -152,7 +155,8 @@ int nsk_row_pass(const nsk_plan *p, const void *spec_in, void *spec_out,
 int nsk_rollout(const nsk_plan *p, const float *u0, float *u_final,
                 nsk_slot *slots, int steps, int scheme, int objective,
                 double *J_out, void *spec0, void *spec1, double *jpart,
-                float *ckpt, int ckpt_every, int use_graph, void *stream);
+                int jcap, float *ckpt, int ckpt_every, int use_graph,
+                void *stream);
 
 /* ---- gradient (SingleCoreNSSolver.jit_gradient, NS_solver.py:725) ----- */
 /* Forward rollout with checkpoints every `ckpt_every` steps, then the

@@ -69,7 +69,7 @@ _SIGS = {
     "nsk_poisson": ([_P, _vp, _vp, _vp, _vp], _i),
     "nsk_project": ([_P, _vp, _vp, _vp, _vp], _i),
     "nsk_step": ([_P, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp], _i),
-    "nsk_rollout": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
+    "nsk_rollout": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp], _i),
     "nsk_rollout_grad": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                           _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
     "nsk_admit": ([_P, _vp, _vp, _i, _i, _vp, _vp, _i, _vp], _i),

@@ -611,7 +611,8 @@ int nsk_row_pass(const nsk_plan* p, const void* spec_in, void* spec_out, const n
 
 static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
                         int scheme, int objective, double* J_out, void* spec0, void* spec1,
-                        double* jpart, float* ckpt, int ckpt_every, bool use_graph, hipStream_t st) {
+                        double* jpart, int jcap, float* ckpt, int ckpt_every, bool use_graph,
+                        hipStream_t st) {
   const long long NN2 = 2LL * p->N * p->N;
   if (steps == 0) {
     HIP_TRY(hipMemcpyAsync(u_final, u0, sizeof(float) * NN2 * p->batch, hipMemcpyDeviceToDevice, st));
@@ -628,7 +629,7 @@ static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_
   r.dst_spec = (float4*)spec0;
   r.require = REQ_ACTIVE;
   r.part = jpart;
-  r.pcap = steps;
+  r.pcap = jcap;
   r.objective = objective;
   if (ckpt) {
     r.ustore = ckpt;
@@ -639,10 +640,10 @@ static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_
   HIP_TRY(launch_row(p->N, RK_STEP_FROM_REAL, scheme, r, p->batch, st));
   void* cur = spec0;
   void* nxt = spec1;
-  int rc = run_steps(p, slots, steps, scheme, objective, &cur, &nxt, u_final, jpart, steps, ckpt,
+  int rc = run_steps(p, slots, steps, scheme, objective, &cur, &nxt, u_final, jpart, jcap, ckpt,
                      ckpt_every, ckpt_cap, use_graph, st);
   if (rc) return rc;
-  if (J_out) HIP_TRY(finalize(p, slots, jpart, steps, 1, J_out, st));
+  if (J_out) HIP_TRY(finalize(p, slots, jpart, jcap, 1, J_out, st));
   return NSK_OK;
 }
 
@@ -657,15 +658,17 @@ static int check_common(const nsk_plan* p, int steps, int scheme, int objective)
 
 int nsk_rollout(const nsk_plan* p, const float* u0, float* u_final, nsk_slot* slots, int steps,
                 int scheme, int objective, double* J_out, void* spec0, void* spec1, double* jpart,
-                float* ckpt, int ckpt_every, int use_graph, void* stream) {
+                int jcap, float* ckpt, int ckpt_every, int use_graph, void* stream) {
   CHECK_PLAN(p);
   int rc = check_common(p, steps, scheme, objective);
   if (rc) return rc;
   if (!u0 || !u_final || !slots || !spec0 || !spec1 || (steps > 0 && !jpart))
     return fail(NSK_E_INVAL, "nsk_rollout: NULL buffer");
+  if (jcap == 0) jcap = steps;
+  if (jcap < steps) return fail(NSK_E_INVAL, "jcap (%d) must be >= steps (%d) or 0", jcap, steps);
   if (ckpt && ckpt_every < 1) return fail(NSK_E_INVAL, "ckpt_every must be >= 1");
-  return rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
-                      ckpt_every, use_graph != 0, (hipStream_t)stream);
+  return rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, jcap,
+                      ckpt, ckpt_every, use_graph != 0, (hipStream_t)stream);
 }
 
 // One chunk of the reverse sweep when every state is checkpointed: steps
@@ -866,8 +869,8 @@ int nsk_rollout_grad(const nsk_plan* p, const float* u0, float* u_final, nsk_slo
       (!seg && ckpt_every > 1) || !dtheta_out)
     return fail(NSK_E_INVAL, "nsk_rollout_grad: NULL buffer");
   hipStream_t st = (hipStream_t)stream;
-  rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, ckpt,
-                    ckpt_every, use_graph != 0, st);
+  rc = rollout_impl(p, u0, u_final, slots, steps, scheme, objective, J_out, spec0, spec1, jpart, steps,
+                    ckpt, ckpt_every, use_graph != 0, st);
   if (rc) return rc;
   return adjoint_impl(p, slots, steps, scheme, objective, 1.0f, nullptr, dtheta_out, du0_out, spec0, spec1,
                       spec2, spec3, gpart, ckpt, seg, ckpt_every, use_graph != 0, st);

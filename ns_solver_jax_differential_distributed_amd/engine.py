@@ -128,7 +128,12 @@ class RolloutEngine:
         u0 = self.field(u0)
         slots = self.slots(nu, dt, theta, counts)
         nb = self.bands()
-        jpart = self.buf("jpart", (self.batch * max(steps, 1) * nb,), torch.float64)
+        # partial sums for a capacity of whole multiples of JCAP_QUANTUM steps
+        # that never shrinks: the replayed step graphs depend on the capacity
+        # and the buffer, so rollouts of different lengths share them
+        self._jcap = max(getattr(self, "_jcap", 0), -(-max(steps, 1) // self.JCAP_QUANTUM) * self.JCAP_QUANTUM)
+        jcap = self._jcap
+        jpart = self.buf("jpart", (self.batch * jcap * nb,), torch.float64)
         J = torch.empty((self.batch,), dtype=torch.float64, device=u0.device)
         u_final = torch.empty_like(u0)
         ckpt = None
@@ -140,7 +145,7 @@ class RolloutEngine:
         _lib.check(_lib.lib().nsk_rollout(
             ctypes.byref(self.plan), u0.data_ptr(), u_final.data_ptr(), slots.data_ptr(), steps,
             _lib.SCHEME[self.scheme], _lib.OBJECTIVE[self.objective], J.data_ptr(),
-            self.spec(0).data_ptr(), self.spec(1).data_ptr(), jpart.data_ptr(),
+            self.spec(0).data_ptr(), self.spec(1).data_ptr(), jpart.data_ptr(), jcap,
             None if ckpt is None else ckpt.data_ptr(), k, int(self.use_graph and (ckpt is None or graph_ckpt)),
             _lib.stream_ptr()), "rollout")
         return J, u_final, ckpt
@@ -179,6 +184,7 @@ class RolloutEngine:
         return dth, du0
 
     AUTO_CKPT_BYTES = 8 << 30
+    JCAP_QUANTUM = 1024
 
     def resolve_ckpt(self, ckpt_every, steps: int) -> int:
         """The checkpoint interval for ``ckpt_every``: an int as given, or for

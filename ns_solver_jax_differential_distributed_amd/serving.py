@@ -93,6 +93,12 @@ class HBMBudget:
         self.used = max(0, self.used - nbytes)
 
 
+def _partials_per_step(N: int) -> int:
+    """Objective partial sums per slot and step at grid size N."""
+    p = _lib.make_plan(N, 1.0, 0.5, 0.1, 1.0, 1)
+    return int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), 1))
+
+
 def _u0_for(req: SolveRequest, L: float) -> torch.Tensor:
     from . import workloads
     if req.u0 is None:
@@ -241,8 +247,14 @@ class _GradLane:
 
     @staticmethod
     def group_bytes(N: int, B: int, steps: int, k: int) -> int:
+        """Device bytes of a group: four spectra, u0, u_final and du0, the
+        checkpoints and recomputation segment, the objective partials
+        (capacity rounded up as the engine does) and the gradient's."""
         ncheck = (steps + k - 1) // k
-        return B * (8 * N * N * (4 + 1 + 1) + 8 * N * N * (ncheck + k) + 16 * steps * 256) + 1024
+        nb = _partials_per_step(N)
+        q = RolloutEngine.JCAP_QUANTUM
+        jcap = -(-max(steps, 1) // q) * q
+        return B * (8 * N * N * (4 + 1 + 1) + 8 * N * N * (ncheck + k) + 8 * nb * (jcap + 2 * steps)) + 1024
 
     def _enqueue(self, reqs: List[SolveRequest], ckpt_every: int):
         B = len(reqs)
