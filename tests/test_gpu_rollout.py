@@ -185,6 +185,23 @@ def test_graph_replayed_gradient_equals_eager(scheme):
         torch.cuda.synchronize()
 
 
+def test_rollouts_of_different_lengths_share_graphs_and_agree():
+    """One solver, rollouts of 40, 100 and again 40 steps (the step graphs are
+    cached per partial-sum capacity, not per length): each equals a fresh
+    solver's rollout of that length bit for bit, and its J the sum over its
+    own steps."""
+    N, L, nu, dt = 64, 1.0, 0.01, 1e-3
+    rng = np.random.default_rng(5)
+    u0 = torch.from_numpy((0.3 * rng.standard_normal((2, N, N))).astype(np.float32))
+    theta = [1.2, 0.7]
+    s = solver(N, L, nu, dt, 40)
+    for steps in (40, 100, 40):
+        J, uf = s.rollout(u0, theta, steps=steps)
+        J1, uf1 = solver(N, L, nu, dt, steps).rollout(u0, theta)
+        assert torch.equal(uf, uf1) and float(J) == float(J1)
+    assert s.engine._jcap == 1024
+
+
 def test_jit_gradient_matches_jax_reverse_mode():
     """JAX's gradient at theta=[1,0.5] from the reference's SGD(lr=0.1)
     trajectory (studies.py:890-918): g = (theta_0 - theta_1)/0.1;
