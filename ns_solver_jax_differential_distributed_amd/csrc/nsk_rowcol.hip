@@ -48,6 +48,12 @@
 // forward with checkpoints 240 -> 195 us per step with non-temporal stores).
 // Non-temporal loads help at N=4096 (reverse sweep 263 -> 250 us per step)
 // but not at N=1024, where neighbouring rows reuse them through L2.
+#ifndef NSK_CKPT_ST_AUX
+#define NSK_CKPT_ST_AUX 2
+#endif
+#ifndef NSK_CKPT_LD_AUX
+#define NSK_CKPT_LD_AUX (NSK_N >= 4096 ? 2 : 0)
+#endif
 // cache policy of the row pass's spectrum loads.  Its input buffer is dead
 // once read (the next row pass overwrites it), but non-temporal loads are
 // slower (N=4096 row pass 83 -> 94 us): neighbouring bands re-read each
@@ -55,11 +61,11 @@
 #ifndef NSK_ROW_IN_AUX
 #define NSK_ROW_IN_AUX 0
 #endif
-#ifndef NSK_CKPT_ST_AUX
-#define NSK_CKPT_ST_AUX 2
-#endif
-#ifndef NSK_CKPT_LD_AUX
-#define NSK_CKPT_LD_AUX (NSK_N >= 4096 ? 2 : 0)
+// points of the reverse sweep's per-point loop between scheduling fences.
+// Wider spacing lets the scheduler hoist more checkpoint loads, but gains
+// nothing (8: same times; 16: N=1024 reverse sweep 107 -> 116 us per step).
+#ifndef NSK_ADJ_FENCE
+#define NSK_ADJ_FENCE 4
 #endif
 
 #define NSK_CAT2(a, b) a##b
@@ -611,7 +617,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
         acc0 += grow * dA;
         acc1 += grow * dB;
         rp[m] = v;
-        if ((m & 3) == 3) NSK_SCHED_FENCE();
+        if ((m % NSK_ADJ_FENCE) == NSK_ADJ_FENCE - 1) NSK_SCHED_FENCE();
       }
     }
     NSK_SCHED_FENCE();
