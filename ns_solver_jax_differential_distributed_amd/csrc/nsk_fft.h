@@ -196,8 +196,15 @@ constexpr int fft_twoff(int N, int E, int s) {
 // (P*(R+1) <= 1024 entries; row k = b mod P holds w^(j*k), j < R, padded to
 // R+1 entries so that 16 lanes with consecutive k hit distinct banks), and
 // in registers otherwise.
+// Passes whose twiddle table has at most this many entries read it from
+// LDS, the others keep this thread's twiddles in registers.  All in
+// registers (0) costs 30 more VGPRs and is slower (N=4096 row pass 83 -> 86,
+// column pass 90 -> 93 us).
+#ifndef NSK_TW_LDS_MAX
+#define NSK_TW_LDS_MAX 1024
+#endif
 constexpr bool fft_tw_lds(int N, int E, int s) {
-  return s > 0 && fft_p(N, E, s) * (fft_radix(N, E, s) + 1) <= 1024;
+  return s > 0 && fft_p(N, E, s) * (fft_radix(N, E, s) + 1) <= NSK_TW_LDS_MAX;
 }
 constexpr int fft_nreg(int N, int E, int s) { return fft_tw_lds(N, E, s) ? 0 : fft_ntw(N, E, s); }
 constexpr int fft_regoff(int N, int E, int s) {

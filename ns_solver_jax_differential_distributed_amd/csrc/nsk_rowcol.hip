@@ -61,6 +61,12 @@
 #ifndef NSK_ROW_IN_AUX
 #define NSK_ROW_IN_AUX 0
 #endif
+// y-neighbours inside a wave by DPP wave shifts (1) or by ds_bpermute (0).
+// The shifts are VALU moves and leave the LDS to the transforms' exchanges:
+// N=4096 row pass 84.5 -> 82 us.
+#ifndef NSK_ROW_DPP
+#define NSK_ROW_DPP 1
+#endif
 // points of the reverse sweep's per-point loop between scheduling fences.
 // Wider spacing lets the scheduler hoist more checkpoint loads, but gains
 // nothing (8: same times; 16: N=1024 reverse sweep 107 -> 116 us per step).
@@ -200,6 +206,15 @@ __device__ __forceinline__ int zslot(int y) {
 
 __device__ __forceinline__ float2 shfl2(float2 v, int src) {
   return make_float2(__shfl(v.x, src), __shfl(v.y, src));
+}
+
+// Whole-wave shift by one lane through DPP (a VALU move, no LDS access):
+// CTRL 0x138 (wave_shr:1) gives lane i the value of lane i-1, 0x130
+// (wave_shl:1) the value of lane i+1; the lane without a source gets 0.
+template <int CTRL>
+__device__ __forceinline__ float2 wave_shift2(float2 v) {
+  return make_float2(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), CTRL, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), CTRL, 0xF, 0xF, false)));
 }
 
 // Raw buffer view of one slot's array: a per-thread 32-bit byte offset
@@ -486,8 +501,13 @@ __global__ void __launch_bounds__(RowK<N>::T)
       u = (t != TL - 1) ? u0 : u1;
     } else {
       const int w = t >> 6, par = x & 1;
+#if NSK_ROW_DPP
+      d = wave_shift2<0x138>(c[m]);
+      u = wave_shift2<0x130>(c[m]);
+#else
       d = shfl2(c[m], sd);
       u = shfl2(c[m], su);
+#endif
       // lane 0: thread t-1 (wave w-1, lane 63), or for t = 0 the last
       // thread's point m-1; lane 63: thread t+1 (wave w+1, lane 0), or for
       // t = TL-1 thread 0's point m+1
