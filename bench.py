@@ -19,6 +19,8 @@ Also reported (one JSON line):
                 number of steps of the same workload
   accuracy      the GPU result against the FP64 oracle over a few steps of
                 the same workload (relative L2 of u, J error, divergence)
+  slab_share    one device's share of a slab-decomposed step (rows N/P, column
+                pairs N/2P) for P in {1, 2, 4, 8}: launch times and efficiency
   sweep         cell-updates/s over N in {1024, 2048, 4096} x {central,
                 upwind} x nu in {1e-4, 1e-3}
   batched, gradient, serving
@@ -170,6 +172,50 @@ def _timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def slab_shares(solver, reps, parts=(1, 2, 4, 8)):
+    """One device's share of a slab-decomposed step at this workload's N:
+    the row pass on rows [0, N/P) and the column pass on spectral column
+    pairs [0, N/(2P)) (nsk_row_pass_rows / nsk_column_pass_cols), average
+    launch time in us with events on the launch stream, and the share's
+    efficiency (whole-field time / P over the share's time)."""
+    import torch
+    from ns_solver_jax_differential_distributed_amd import _lib
+    lib = _lib.lib()
+    p = solver._plan()
+    N = p.N
+    s0, s1 = solver._ws["spec0"], solver._ws["spec1"]
+    slots = _lib.make_slots(solver.phys.nu, solver.time.dt, 1.0, 0.5, 0, 1 << 30)
+    st = _lib.stream_ptr()
+    scheme = _lib.SCHEME[solver.scheme]
+
+    def timed(launch):
+        for _ in range(5):
+            launch()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            launch()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    out = {}
+    for P in parts:
+        nx, nc = N // P, N // 2 // P
+        row = timed(lambda: _lib.check(lib.nsk_row_pass_rows(ctypes.byref(p), 0, nx, s0.data_ptr(), s1.data_ptr(),
+                                                             slots.data_ptr(), 0, scheme, 0, None, 0, st), "row"))
+        col = timed(lambda: _lib.check(lib.nsk_column_pass_cols(ctypes.byref(p), 0, nc, s0.data_ptr(), 0, st),
+                                       "column"))
+        out[f"P{P}"] = {"rows": nx, "column_pairs": nc, "row_us": round(row, 2), "column_us": round(col, 2)}
+    for P in parts:
+        o = out[f"P{P}"]
+        o["row_eff"] = round(out["P1"]["row_us"] / P / o["row_us"], 3)
+        o["column_eff"] = round(out["P1"]["column_us"] / P / o["column_us"], 3)
+    return out
 
 
 def leg_sweep(steps=200, warmup=40):
@@ -380,6 +426,10 @@ def main():
         line["cpu_baseline"] = cpu_baseline(u0, theta, N, L, nu, dt, args.scheme, args.cpu_seconds)
         line["accuracy"] = accuracy(solver, u0, u0_dev, theta, N, L, nu, dt, args.scheme, args.accuracy_steps)
     if not args.no_extras:
+        try:
+            line["slab_share"] = slab_shares(solver, args.reps)
+        except Exception as e:
+            line["slab_share"] = {"error": f"{type(e).__name__}: {e}"}
         for key, fn in (("sweep", leg_sweep), ("batched", leg_batched), ("gradient", leg_gradient),
                         ("serving", leg_serving)):
             try:

@@ -139,6 +139,30 @@ int nsk_row_pass(const nsk_plan *p, const void *spec_in, void *spec_out,
                  const nsk_slot *slots, int step_off, int scheme, int objective,
                  double *jpart, int jcap, void *stream);
 
+/* ---- one device's share of a slab-decomposed step ----------------------
+ * The reference's DistributedNSSolver (NS_solver.py:759-883) splits the
+ * field into x-slabs, one per device.  With the row-spectral state the row
+ * pass of a device covers its own rows (plus one halo row on each side, read
+ * here from the same buffer), and after an all-to-all of the spectrum the
+ * column pass covers a range of spectral column pairs over all x.  These two
+ * calls run exactly that share on this GPU's buffer of the whole field, so
+ * that the per-device kernel cost is measured at its real shape; they
+ * compute the same values as nsk_row_pass / nsk_column_pass restricted to
+ * the range and leave everything outside it untouched.
+ * Rows x0 .. x0+nx-1 of every slot; x0 and nx multiples of
+ * nsk_row_quantum(N) (16 at N = 4096). */
+int nsk_row_pass_rows(const nsk_plan *p, int x0, int nx, const void *spec_in,
+                      void *spec_out, const nsk_slot *slots, int step_off,
+                      int scheme, int objective, double *jpart, int jcap,
+                      void *stream);
+/* Spectral column pairs c0 .. c0+nc-1 (of N/2) of every slot, in place;
+ * c0 and nc multiples of nsk_col_quantum(N) (2 at N = 4096). */
+int nsk_column_pass_cols(const nsk_plan *p, int c0, int nc, void *spec, int mode,
+                         void *stream);
+/* Granularities of the two ranges for grid size N (0 if N is unsupported). */
+int nsk_row_quantum(int N);
+int nsk_col_quantum(int N);
+
 /* ---- rollout (SingleCoreNSSolver.rollout, NS_solver.py:732-750) ------- */
 /* Runs the Chorin steps of every slot from u0: slot b takes
  * slots[b].nsteps <= steps steps (its final state is written when it gets

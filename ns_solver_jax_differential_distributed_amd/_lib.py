@@ -32,6 +32,7 @@ EXPORTED = (
     "nsk_advect", "nsk_forcing", "nsk_poisson", "nsk_project", "nsk_step",
     "nsk_rollout", "nsk_rollout_grad", "nsk_admit", "nsk_advance",
     "nsk_finalize", "nsk_column_pass", "nsk_row_pass", "nsk_rollout_adjoint",
+    "nsk_row_pass_rows", "nsk_column_pass_cols", "nsk_row_quantum", "nsk_col_quantum",
 )
 
 
@@ -77,6 +78,10 @@ _SIGS = {
     "nsk_finalize": ([_P, _vp, _vp, _i, _i, _vp, _vp], _i),
     "nsk_column_pass": ([_P, _vp, _i, _vp], _i),
     "nsk_row_pass": ([_P, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp], _i),
+    "nsk_row_pass_rows": ([_P, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp, _i, _vp], _i),
+    "nsk_column_pass_cols": ([_P, _i, _i, _vp, _i, _vp], _i),
+    "nsk_row_quantum": ([_i], _i),
+    "nsk_col_quantum": ([_i], _i),
     "nsk_rollout_adjoint": ([_P, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                              _vp, _vp, _i, _i, _vp], _i),
 }

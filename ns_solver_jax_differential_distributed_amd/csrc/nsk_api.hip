@@ -207,8 +207,10 @@ __global__ void k_stencil(StencilArgs a) {
 }
 
 // out[b*ncomp + c] = sum_{n < nsteps_b, band} part[b][n][band][c], in a
-// fixed order (thread-strided partial sums, then a fixed tree).
-constexpr int kFinT = 1024;
+// fixed order (thread-strided partial sums, then a fixed tree).  Each
+// thread issues kFinU loads before adding them in the same order as one at
+// a time, so the sum keeps its bits and the loads overlap.
+constexpr int kFinT = 1024, kFinU = 8;
 __global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap, int nb, int ncomp,
                                                     const nsk_slot* slots, double* out) {
   __shared__ double red[kFinT];
@@ -218,7 +220,15 @@ __global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap
   const size_t total = (size_t)nsteps * nb;
   const double* base = part + (size_t)b * pcap * nb * ncomp;
   double s = 0.0;
-  for (size_t k = tid; k < total; k += kFinT) s += base[k * ncomp + c];
+  size_t k = tid;
+  for (; k + (size_t)(kFinU - 1) * kFinT < total; k += (size_t)kFinU * kFinT) {
+    double v[kFinU];
+#pragma unroll
+    for (int j = 0; j < kFinU; ++j) v[j] = base[(k + (size_t)j * kFinT) * ncomp + c];
+#pragma unroll
+    for (int j = 0; j < kFinU; ++j) s += v[j];
+  }
+  for (; k < total; k += kFinT) s += base[k * ncomp + c];
   red[tid] = s;
   __syncthreads();
   for (int w = kFinT / 2; w > 0; w >>= 1) {
@@ -606,6 +616,59 @@ int nsk_row_pass(const nsk_plan* p, const void* spec_in, void* spec_out, const n
   r.pcap = jcap;
   r.objective = objective;
   HIP_TRY(launch_row(p->N, RK_STEP_ROLL, scheme, r, p->batch, (hipStream_t)stream));
+  return NSK_OK;
+}
+
+int nsk_row_pass_rows(const nsk_plan* p, int x0, int nx, const void* spec_in, void* spec_out,
+                      const nsk_slot* slots, int step_off, int scheme, int objective, double* jpart,
+                      int jcap, void* stream) {
+  // arguments first: a bad range is reported without touching the GPU
+  int rc = check_plan(p);
+  if (rc) return rc;
+  const int q = row_quantum(p->N);
+  if (x0 < 0 || nx <= 0 || x0 + nx > p->N || x0 % q != 0 || nx % q != 0)
+    return fail(NSK_E_INVAL, "row range [%d, %d) must lie in [0, %d) with both ends multiples of %d", x0,
+                x0 + nx, p->N, q);
+  if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
+    return fail(NSK_E_INVAL, "unknown advection scheme %d", scheme);
+  if (objective != NSK_OBJ_WORK && objective != NSK_OBJ_MEAN_KE)
+    return fail(NSK_E_INVAL, "unknown objective %d", objective);
+  if (!spec_in || !spec_out || !slots) return fail(NSK_E_INVAL, "nsk_row_pass_rows: NULL buffer");
+  if (spec_in == spec_out) return fail(NSK_E_INVAL, "nsk_row_pass_rows: input and output spectra must differ");
+  rc = ensure_init();
+  if (rc) return rc;
+  RowArgs r = row_args(p, slots);
+  r.src_spec = (const float4*)spec_in;
+  r.dst_spec = (float4*)spec_out;
+  r.step_off = step_off;
+  r.require = REQ_ACTIVE;
+  r.part = jpart;
+  r.pcap = jcap;
+  r.objective = objective;
+  r.x0 = x0;
+  r.nx = nx;
+  HIP_TRY(launch_row(p->N, RK_STEP_ROLL, scheme, r, p->batch, (hipStream_t)stream));
+  return NSK_OK;
+}
+
+int nsk_row_quantum(int N) { return supported_n(N) ? row_quantum(N) : 0; }
+int nsk_col_quantum(int N) { return supported_n(N) ? col_quantum(N) : 0; }
+
+int nsk_column_pass_cols(const nsk_plan* p, int c0, int nc, void* spec, int mode, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  const int q = col_quantum(p->N);
+  if (c0 < 0 || nc <= 0 || c0 + nc > p->N / 2 || c0 % q != 0 || nc % q != 0)
+    return fail(NSK_E_INVAL, "column range [%d, %d) must lie in [0, %d) with both ends multiples of %d", c0,
+                c0 + nc, p->N / 2, q);
+  if (!spec) return fail(NSK_E_INVAL, "nsk_column_pass_cols: NULL buffer");
+  if (mode != COL_PROJECT && mode != COL_POISSON) return fail(NSK_E_INVAL, "mode must be 0 or 1, got %d", mode);
+  rc = ensure_init();
+  if (rc) return rc;
+  ColArgs c = col_args(p, spec, nullptr, 0, REQ_ANY);
+  c.kb0 = c0 / q;
+  c.nblk = nc / q;
+  HIP_TRY(launch_col(p->N, mode, c, p->batch, (hipStream_t)stream));
   return NSK_OK;
 }
 

@@ -49,6 +49,7 @@ struct RowArgs {
   int us_base;
   float jseed;
   int rows;               // rows per stream (set by the launcher)
+  int x0, nx;             // rows x0 .. x0+nx-1 of every slot (nx = 0: all)
 };
 
 struct ColArgs {
@@ -57,6 +58,7 @@ struct ColArgs {
   const nsk_slot* slots;
   int step_off;
   int require;   // REQ_ANY or REQ_ACTIVE
+  int kb0, nblk; // column blocks kb0 .. kb0+nblk-1 (nblk = 0: all)
 };
 
 struct StencilArgs {
@@ -86,7 +88,10 @@ struct RowGeo;
 #ifndef NSK_ROW_DB
 #define NSK_ROW_DB 1
 #endif
-template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = 8, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
+#ifndef NSK_ROW_RMIN4096
+#define NSK_ROW_RMIN4096 2
+#endif
+template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = NSK_ROW_RMIN4096, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
 template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
 template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
 template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
@@ -101,13 +106,13 @@ template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, RMIN = 1, RM
 // fill the GPU (about 2048 resident waves; 1024 below N = 1024, where a
 // wave holds several streams), as few halo rows as that allows.
 template <int N>
-constexpr int pick_rows(int batch) {
+constexpr int pick_rows(int batch, int nrows = N) {
   using G = RowGeo<N>;
   constexpr int EE = N < G::E ? N : G::E;
   constexpr int TL = N / EE;
   const long target = N >= 1024 ? 2048 : 1024;
-  // waves = batch * (N / rows) * TL / 64
-  const long num = (long)batch * N * TL;
+  // waves = batch * (nrows / rows) * TL / 64
+  const long num = (long)batch * nrows * TL;
   int rows = G::RMIN;
   while (rows * 2 <= G::RMAX && num / (64L * rows * 2) >= target) rows *= 2;
   return rows;
@@ -179,6 +184,27 @@ inline int row_parts(int N) {
     case 16: return row_parts_c<16>();
     case 8: return row_parts_c<8>();
     default: return 1;
+  }
+}
+
+// Granularity of a row range (rows of one workgroup at the longest band) and
+// of a column range (spectral column pairs of one column block).
+inline int row_quantum(int N) {
+  switch (N) {
+#define NSK_RQ(n) case n: return RowGeo<n>::LG * RowGeo<n>::RMAX;
+    NSK_RQ(4096) NSK_RQ(2048) NSK_RQ(1024) NSK_RQ(512) NSK_RQ(256)
+    NSK_RQ(128) NSK_RQ(64) NSK_RQ(32) NSK_RQ(16) NSK_RQ(8)
+#undef NSK_RQ
+    default: return 0;
+  }
+}
+inline int col_quantum(int N) {
+  switch (N) {
+#define NSK_CQ(n) case n: return ColGeo<n>::W;
+    NSK_CQ(4096) NSK_CQ(2048) NSK_CQ(1024) NSK_CQ(512) NSK_CQ(256)
+    NSK_CQ(128) NSK_CQ(64) NSK_CQ(32) NSK_CQ(16) NSK_CQ(8)
+#undef NSK_CQ
+    default: return 0;
   }
 }
 

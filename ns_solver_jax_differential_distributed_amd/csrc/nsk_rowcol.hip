@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
   const bool two_out = a.dst_nc > 1;
 
   const int rows = a.rows;
-  const int xs = (band * LG + g) * rows;  // first row of this stream
+  const int xs = a.x0 + (band * LG + g) * rows;  // first row of this stream
 
   Twid<N, E> tw;
   tw.load(t, s_tw, tid, T);
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(RowK<N>::T)
 
   // ---- per-row forcing (f_x depends on x only, f_y = 0)
   if (tid < LG * rows) {
-    const int x = band * LG * rows + tid;
+    const int x = a.x0 + band * LG * rows + tid;
     const float xc = p.Lf * ((float)x / (float)N);
     s_f[tid] = FFIELD ? 0.0f : forcing_x(sl.A, sl.B, tm, xc, p);
     if constexpr (OP == OP_ADJ) {
@@ -706,13 +706,14 @@ __global__ void __launch_bounds__(ColK<N>::T) k_col(ColArgs a) {
   __shared__ float2 s_tw[RF<N, E>::NLDS > 0 ? RF<N, E>::NLDS : 1];
   const int tid = threadIdx.x, kw = tid % W, t = tid / W, b = blockIdx.y;
   const int nwg = gridDim.x;
-  const int w0 = xcd_remap(blockIdx.x, nwg);
+  const int w0 = a.kb0 + xcd_remap(blockIdx.x, nwg);
+  const int kend = a.nblk > 0 ? a.kb0 + a.nblk : NBLK;
   if (a.require == REQ_ACTIVE) {
     const nsk_slot sl = a.slots[b];
     const int n = sl.step + a.step_off;
     if (n < 0 || n >= sl.nsteps) return;
   }
-  if (w0 >= NBLK) return;
+  if (w0 >= kend) return;
   float4* H = a.spec + (size_t)b * N * HN;
   const nsk_plan& p = a.plan;
   const float scale = 1.0f / ((float)N * (float)N);
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(ColK<N>::T) k_col(ColArgs a) {
   };
   if (PF) fetch(w0);
 #pragma unroll 1
-  for (int kb = w0; kb < NBLK; kb = PERSIST ? kb + nwg : NBLK) {
+  for (int kb = w0; kb < kend; kb = PERSIST ? kb + nwg : kend) {
     const int c = kb * W + kw;
     if (!PF) fetch(kb);
     float2 v[2][EE];
@@ -861,7 +862,8 @@ __global__ void __launch_bounds__(ColXY<N>::T) k_colxy(ColArgs a) {
   const int xy = tid & 1, kw = (tid >> 1) % W, t = tid / (2 * W);
   const int b = blockIdx.y;
   const int nwg = gridDim.x;
-  const int w0 = xcd_remap(blockIdx.x, nwg);
+  const int w0 = a.kb0 + xcd_remap(blockIdx.x, nwg);
+  const int kend = a.nblk > 0 ? a.kb0 + a.nblk : NBLK;
   if (a.require == REQ_ACTIVE) {
     const nsk_slot sl = a.slots[b];
     const int n = sl.step + a.step_off;
@@ -886,7 +888,7 @@ __global__ void __launch_bounds__(ColXY<N>::T) k_colxy(ColArgs a) {
   constexpr bool PERSIST = ColGeo<N>::NWG < NBLK;
 
 #pragma unroll 1
-  for (int kb = w0; kb < NBLK; kb = PERSIST ? kb + nwg : NBLK) {
+  for (int kb = w0; kb < kend; kb = PERSIST ? kb + nwg : kend) {
     const int c = kb * W + kw;
     const unsigned off = hrow + 16u * (unsigned)(kb * W);
     float2 v[1][EE];
@@ -1003,8 +1005,9 @@ namespace {
 template <int KIND, int SCHEME>
 hipError_t go_row(const RowArgs& a0, int batch, hipStream_t s) {
   RowArgs a = a0;
-  a.rows = pick_rows<NN_>(batch);
-  const int bands = NN_ / (RowGeo<NN_>::LG * a.rows);
+  const int nrows = a.nx > 0 ? a.nx : NN_;
+  a.rows = pick_rows<NN_>(batch, nrows);
+  const int bands = nrows / (RowGeo<NN_>::LG * a.rows);
   hipLaunchKernelGGL((k_row<NN_, KIND, SCHEME>), dim3(bands, batch), dim3(RowK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
@@ -1032,10 +1035,11 @@ hipError_t NSK_CAT(launch_row_, NSK_N)(int kind, int scheme, const RowArgs& a, i
 namespace {
 template <int MODE>
 hipError_t go_col(const ColArgs& a, int batch, hipStream_t s) {
+  const int nwg = (a.nblk > 0 && a.nblk < ColGeo<NN_>::NWG) ? a.nblk : ColGeo<NN_>::NWG;
   if constexpr (ColGeo<NN_>::XY)
-    hipLaunchKernelGGL((k_colxy<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColXY<NN_>::T), 0, s, a);
+    hipLaunchKernelGGL((k_colxy<NN_, MODE>), dim3(nwg, batch), dim3(ColXY<NN_>::T), 0, s, a);
   else
-    hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(ColGeo<NN_>::NWG, batch), dim3(ColK<NN_>::T), 0, s, a);
+    hipLaunchKernelGGL((k_col<NN_, MODE>), dim3(nwg, batch), dim3(ColK<NN_>::T), 0, s, a);
   return hipGetLastError();
 }
 

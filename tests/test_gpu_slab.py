@@ -1,0 +1,110 @@
+"""One device's share of a slab-decomposed step (nsk_row_pass_rows,
+nsk_column_pass_cols; the reference's DistributedNSSolver splits x over
+devices, NS_solver.py:759-883): the range launches compute exactly what the
+whole-field passes compute on those rows / column pairs, bit for bit, leave
+the rest of the buffer untouched, and P shares together are the whole step."""
+import ctypes
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def L():
+    from ns_solver_jax_differential_distributed_amd import _lib
+    return _lib
+
+
+def setup(N, B, seed):
+    _lib = L()
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    spec = (0.05 * torch.randn((B, N, N // 2, 4), generator=g)).cuda()
+    slots = _lib.make_slots(1e-3, 2e-3, 0.9, 0.7, 0, 1 << 30, batch=B)
+    p = _lib.make_plan(N, 2 * math.pi, batch=B)
+    npart = int(_lib.load_library().nsk_partials_per_slot(ctypes.byref(p), 1))
+    return p, spec, slots, npart
+
+
+def row_full(p, src, slots, npart, B):
+    _lib = L()
+    out = torch.full_like(src, float("nan"))
+    jp = torch.full((B, npart), float("nan"), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().nsk_row_pass(ctypes.byref(p), src.data_ptr(), out.data_ptr(), slots.data_ptr(),
+                                       0, 0, 0, jp.data_ptr(), 1, _lib.stream_ptr()), "row")
+    return out, jp
+
+
+def row_range(p, src, out, slots, jp, x0, nx):
+    _lib = L()
+    _lib.check(_lib.lib().nsk_row_pass_rows(ctypes.byref(p), x0, nx, src.data_ptr(), out.data_ptr(),
+                                            slots.data_ptr(), 0, 0, 0, jp.data_ptr(), 1,
+                                            _lib.stream_ptr()), "row range")
+
+
+def col_full(p, spec):
+    _lib = L()
+    _lib.check(_lib.lib().nsk_column_pass(ctypes.byref(p), spec.data_ptr(), 0, _lib.stream_ptr()), "col")
+
+
+def col_range(p, spec, c0, nc):
+    _lib = L()
+    _lib.check(_lib.lib().nsk_column_pass_cols(ctypes.byref(p), c0, nc, spec.data_ptr(), 0,
+                                               _lib.stream_ptr()), "col range")
+
+
+def same(a, b):
+    """Bitwise equality, NaN sentinels included."""
+    return torch.equal(a.view(torch.int32), b.view(torch.int32))
+
+
+@pytest.mark.parametrize("N,B,parts,r", [(4096, 1, 8, 3), (2048, 1, 8, 7), (1024, 2, 4, 0), (256, 3, 2, 1),
+                                             (128, 2, 2, 1)])
+def test_one_share_equals_the_whole_field_passes_there(N, B, parts, r):
+    p, spec, slots, npart = setup(N, B, seed=N + r)
+    full, jfull = row_full(p, spec, slots, npart, B)
+    nx = N // parts
+    x0 = r * nx
+    out = torch.full_like(spec, float("nan"))
+    jp = torch.full((B, npart), float("nan"), dtype=torch.float64, device="cuda")
+    row_range(p, spec, out, slots, jp, x0, nx)
+    torch.cuda.synchronize()
+    assert same(out[:, x0:x0 + nx], full[:, x0:x0 + nx])
+    assert torch.isnan(out[:, :x0]).all() and torch.isnan(out[:, x0 + nx:]).all()
+    # objective partials: the share writes exactly the entries of its rows
+    per_row = npart / N
+    e0, e1 = int(x0 * per_row), int((x0 + nx) * per_row)
+    assert same(jp[:, e0:e1], jfull[:, e0:e1])
+    assert torch.isnan(jp[:, :e0]).all() and torch.isnan(jp[:, e1:]).all()
+
+    # column pass on the share's column pairs
+    HN = N // 2
+    nc = HN // parts
+    c0 = r * nc
+    ref = full.clone()
+    col_full(p, ref)
+    part = full.clone()
+    col_range(p, part, c0, nc)
+    torch.cuda.synchronize()
+    assert same(part[:, :, c0:c0 + nc], ref[:, :, c0:c0 + nc])
+    assert same(part[:, :, :c0], full[:, :, :c0]) and same(part[:, :, c0 + nc:], full[:, :, c0 + nc:])
+
+
+@pytest.mark.parametrize("N,parts", [(4096, 8), (512, 4)])
+def test_all_shares_together_are_the_step(N, parts):
+    """P row shares, then P column shares (the all-to-all between them is
+    the identity on one buffer): the same spectrum and partials as one
+    whole-field row pass and column pass."""
+    B = 1
+    p, spec, slots, npart = setup(N, B, seed=7)
+    full, jfull = row_full(p, spec, slots, npart, B)
+    col_full(p, full)
+    out = torch.full_like(spec, float("nan"))
+    jp = torch.full((B, npart), float("nan"), dtype=torch.float64, device="cuda")
+    for r in range(parts):
+        row_range(p, spec, out, slots, jp, r * (N // parts), N // parts)
+    for r in range(parts):
+        col_range(p, out, r * (N // 2 // parts), N // 2 // parts)
+    torch.cuda.synchronize()
+    assert same(out, full) and same(jp, jfull)

@@ -171,6 +171,30 @@ def test_plan_init_host_only():
             _lib.make_plan(64, 1.0, batch=bad)
 
 
+def test_slab_range_checks_need_no_gpu():
+    """One device's share of a slab step (nsk_row_pass_rows /
+    nsk_column_pass_cols): ranges are checked against the granularity before
+    anything touches the GPU."""
+    lib = _lib.load_library()
+    assert lib.nsk_row_quantum(4096) == 16 and lib.nsk_col_quantum(4096) == 2
+    assert lib.nsk_row_quantum(1024) == 16 and lib.nsk_col_quantum(1024) == 4
+    assert lib.nsk_row_quantum(12) == 0 and lib.nsk_col_quantum(8192) == 0
+    p = _lib.make_plan(4096, 2 * math.pi)
+    fake = ctypes.c_void_p(4096)
+    other = ctypes.c_void_p(8192)
+    for x0, nx in ((0, 0), (-16, 32), (8, 16), (0, 24), (4080, 32)):
+        rc = lib.nsk_row_pass_rows(ctypes.byref(p), x0, nx, fake, other, fake, 0, 0, 0, None, 0, None)
+        with pytest.raises(ValueError, match="row range"):
+            _lib.check(rc)
+    for c0, nc in ((0, 0), (1, 2), (0, 3), (2046, 4)):
+        rc = lib.nsk_column_pass_cols(ctypes.byref(p), c0, nc, fake, 0, None)
+        with pytest.raises(ValueError, match="column range"):
+            _lib.check(rc)
+    rc = lib.nsk_row_pass_rows(ctypes.byref(p), 0, 512, fake, fake, fake, 0, 0, 0, None, 0, None)
+    with pytest.raises(ValueError, match="must differ"):
+        _lib.check(rc)
+
+
 def test_slot_record_layout():
     import torch
     rec = _lib.make_slots(0.01, 1e-3, 0.9, 0.4, 5, 100, batch=2, device=torch.device("cpu"))
