@@ -62,8 +62,9 @@
 #define NSK_ROW_IN_AUX 0
 #endif
 // y-neighbours inside a wave by DPP wave shifts (1) or by ds_bpermute (0).
-// The shifts are VALU moves and leave the LDS to the transforms' exchanges:
-// N=4096 row pass 84.5 -> 82 us.
+// The shifts are VALU moves and leave the LDS to the transforms' exchanges
+// (N=4096 row pass 84.5 -> 82 us); the wave-edge value rides in as the
+// shift's `old` operand, which saves the per-point select (82 -> 80.6 us).
 #ifndef NSK_ROW_DPP
 #define NSK_ROW_DPP 1
 #endif
@@ -210,11 +211,14 @@ __device__ __forceinline__ float2 shfl2(float2 v, int src) {
 
 // Whole-wave shift by one lane through DPP (a VALU move, no LDS access):
 // CTRL 0x138 (wave_shr:1) gives lane i the value of lane i-1, 0x130
-// (wave_shl:1) the value of lane i+1; the lane without a source gets 0.
+// (wave_shl:1) the value of lane i+1; the lane without a source (0 or 63)
+// keeps `old` (bound_ctrl off).
 template <int CTRL>
-__device__ __forceinline__ float2 wave_shift2(float2 v) {
-  return make_float2(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), CTRL, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), CTRL, 0xF, 0xF, false)));
+__device__ __forceinline__ float2 wave_shift2(float2 v, float2 old) {
+  return make_float2(__int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old.x), __float_as_int(v.x), CTRL,
+                                                                0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old.y), __float_as_int(v.y), CTRL,
+                                                                0xF, 0xF, false)));
 }
 
 // Raw buffer view of one slot's array: a per-thread 32-bit byte offset
@@ -501,13 +505,6 @@ __global__ void __launch_bounds__(RowK<N>::T)
       u = (t != TL - 1) ? u0 : u1;
     } else {
       const int w = t >> 6, par = x & 1;
-#if NSK_ROW_DPP
-      d = wave_shift2<0x138>(c[m]);
-      u = wave_shift2<0x130>(c[m]);
-#else
-      d = shfl2(c[m], sd);
-      u = shfl2(c[m], su);
-#endif
       // lane 0: thread t-1 (wave w-1, lane 63), or for t = 0 the last
       // thread's point m-1; lane 63: thread t+1 (wave w+1, lane 0), or for
       // t = TL-1 thread 0's point m+1
@@ -515,8 +512,16 @@ __global__ void __launch_bounds__(RowK<N>::T)
                                 : s_edge[((par * NW + NW - 1) * 2 + 1) * EE + (m + EE - 1) % EE];
       const float2 ue = (w < NW - 1) ? s_edge[((par * NW + w + 1) * 2 + 0) * EE + m]
                                      : s_edge[((par * NW) * 2 + 0) * EE + (m + 1) % EE];
+#if NSK_ROW_DPP
+      // the lane without a source keeps `old`: the edge value
+      d = wave_shift2<0x138>(c[m], de);
+      u = wave_shift2<0x130>(c[m], ue);
+#else
+      d = shfl2(c[m], sd);
+      u = shfl2(c[m], su);
       if (lane == 0) d = de;
       if (lane == 63) u = ue;
+#endif
     }
   };
 
