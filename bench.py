@@ -61,6 +61,8 @@ def parse():
                     help="skip the batched / gradient / serving legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 device copies of 256 MiB (scale for HBM byte counters)")
+    ap.add_argument("--dft-gemm", action="store_true",
+                    help="only compare the column pass at small N with the same transforms as FP32 GEMMs")
     return ap.parse_args()
 
 
@@ -172,6 +174,58 @@ def _timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def dft_gemm_compare(reps, cases=((128, 64), (256, 16), (512, 4))):
+    """Small-N check of the alternative of a DFT as a matrix product on the
+    matrix cores: the column pass on B slots of an N x N field (about 1M
+    cells each case) against its two column transforms done as FP32 GEMMs
+    (torch.matmul, hipBLASLt on MFMA): per slot the N complex lines of
+    length N (X and Y of the N/2 column pairs), forward and inverse, each as
+    one real product [2N x 2N] @ [2N x N*B].  The GEMM side leaves out the
+    projection and the separation of the fields, so it is a lower bound on
+    a GEMM-based column pass.  Average launch times from events."""
+    import torch
+    from ns_solver_jax_differential_distributed_amd import _lib
+    lib = _lib.lib()
+    st = _lib.stream_ptr()
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    rows = []
+    g = torch.Generator(device="cpu").manual_seed(0)
+    for N, B in cases:
+        p = _lib.make_plan(N, 2 * math.pi, batch=B)
+        spec = (0.05 * torch.randn((B, N, N // 2, 4), generator=g)).cuda()
+        fft_us = timed(lambda: _lib.check(lib.nsk_column_pass(ctypes.byref(p), spec.data_ptr(), 0, st), "column"))
+        k = torch.arange(N, dtype=torch.float64)
+        ang = 2 * math.pi * torch.outer(k, k) / N
+        c, s_ = torch.cos(ang), torch.sin(ang)
+        fwd = torch.cat([torch.cat([c, s_], 1), torch.cat([-s_, c], 1)], 0).float().cuda()   # e^{-i..}
+        inv = (torch.cat([torch.cat([c, -s_], 1), torch.cat([s_, c], 1)], 0) / N).float().cuda()
+        d = torch.randn((2 * N, N * B), generator=g).cuda()
+        tmp = torch.empty_like(d)
+        out = torch.empty_like(d)
+
+        def gemm():
+            torch.matmul(fwd, d, out=tmp)
+            torch.matmul(inv, tmp, out=out)
+        gemm_us = timed(gemm)
+        flops = 2 * 2 * (2 * N) ** 2 * (N * B)
+        rows.append({"N": N, "slots": B, "cells": N * N * B, "column_pass_us": round(fft_us, 2),
+                     "gemm_transforms_us": round(gemm_us, 2), "gemm_tflops": round(flops / gemm_us / 1e6, 1)})
+    print(json.dumps({"dft_gemm_vs_column_pass": rows}), flush=True)
 
 
 def slab_shares(solver, reps, parts=(1, 2, 4, 8)):
@@ -363,6 +417,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.dft_gemm:
+        dft_gemm_compare(args.reps)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -210,7 +210,7 @@ __global__ void k_stencil(StencilArgs a) {
 // fixed order (thread-strided partial sums, then a fixed tree).  Each
 // thread issues kFinU loads before adding them in the same order as one at
 // a time, so the sum keeps its bits and the loads overlap.
-constexpr int kFinT = 1024, kFinU = 8;
+constexpr int kFinT = 1024, kFinU = 32;
 __global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap, int nb, int ncomp,
                                                     const nsk_slot* slots, double* out) {
   __shared__ double red[kFinT];

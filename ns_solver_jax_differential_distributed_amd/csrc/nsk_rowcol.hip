@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(ColK<N>::T) k_col(ColArgs a) {
         v[1][m] = make_float2(pf[m].y + pf[m].w, pf[m].z - pf[m].x);
       }
     }
-    if (PF && kb + nwg < NBLK) fetch(kb + nwg);
+    if (PF && kb + nwg < kend) fetch(kb + nwg);
     rfft<N, E, -1, 2, false>(v, xb, t, tw);
 
     const float sc = (c == 0) ? scale : 0.5f * scale;
