@@ -63,6 +63,9 @@ def parse():
                     help="only run 10 device copies of 256 MiB (scale for HBM byte counters)")
     ap.add_argument("--dft-gemm", action="store_true",
                     help="only compare the column pass at small N with the same transforms as FP32 GEMMs")
+    ap.add_argument("--profile-gradient", action="store_true",
+                    help="only run rollout + gradient (every state checkpointed) of the workload at --N "
+                         "for --steps steps, three times, for a kernel trace of the gradient path")
     return ap.parse_args()
 
 
@@ -174,6 +177,26 @@ def _timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def profile_gradient(N, steps, nu, scheme, reps=3):
+    """Rollout + discrete adjoint of the bench workload at N, every state
+    checkpointed (k = 1: forward with checkpoint stores, reverse sweep reading
+    them), `reps` times after one warm-up call; prints the time per step."""
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    u0 = W.initial_velocity(N, seed=0, kmax=8)
+    dt = W.cfl_dt(u0, L, 0.3, nu, stable=True, steps=steps)
+    theta = W.theta_batch(1, seed=0)[0]
+    solver = S.SingleCoreNSSolver(S.make_grid(N, L), S.Phys(nu, 0.5, 0.1, 1.0), S.Time(dt, steps),
+                                  advection_scheme=scheme, ckpt_every=1)
+    u0_dev = torch.from_numpy(u0).cuda()
+    solver.rollout_and_grad(u0_dev, theta)
+    t, _ = _timed(lambda: [solver.rollout_and_grad(u0_dev, theta) for _ in range(reps)])
+    print(json.dumps({"profile_gradient": {"N": N, "steps": steps, "scheme": scheme, "ckpt_every": 1,
+                                           "us_per_step": t / reps / steps * 1e6}}), flush=True)
 
 
 def dft_gemm_compare(reps, cases=((128, 64), (256, 16), (512, 4))):
@@ -420,6 +443,9 @@ def main():
         return
     if args.dft_gemm:
         dft_gemm_compare(args.reps)
+        return
+    if args.profile_gradient:
+        profile_gradient(args.N, args.steps, args.nu, args.scheme)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
