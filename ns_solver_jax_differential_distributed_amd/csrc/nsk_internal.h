@@ -102,6 +102,12 @@ template <> struct RowGeo<32> { static constexpr int E = 16, LG = 16, RMIN = 2, 
 template <> struct RowGeo<16> { static constexpr int E = 16, LG = 16, RMIN = 1, RMAX = 1; static constexpr bool DB = true; };
 template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, RMIN = 1, RMAX = 1; static constexpr bool DB = true; };
 
+// Resident waves a row launch aims at for N = 1024 and 2048.  Fewer (longer
+// bands, fewer re-transformed halo rows) lose: one N=2048 field's row pass
+// 28.8 -> 35 us at 1024 waves, 50 us at 512; N=1024 x 8 slots step 94 -> 106 us.
+#ifndef NSK_ROW_WAVES_MID
+#define NSK_ROW_WAVES_MID 2048
+#endif
 // Rows per stream for a launch over `batch` slots: enough workgroups to
 // fill the GPU (about 2048 resident waves; 1024 below N = 1024, where a
 // wave holds several streams), as few halo rows as that allows.
@@ -110,7 +116,7 @@ constexpr int pick_rows(int batch, int nrows = N) {
   using G = RowGeo<N>;
   constexpr int EE = N < G::E ? N : G::E;
   constexpr int TL = N / EE;
-  const long target = N >= 1024 ? 2048 : 1024;
+  const long target = N >= 4096 ? 2048 : (N >= 1024 ? NSK_ROW_WAVES_MID : 1024);
   // waves = batch * (nrows / rows) * TL / 64
   const long num = (long)batch * nrows * TL;
   int rows = G::RMIN;
