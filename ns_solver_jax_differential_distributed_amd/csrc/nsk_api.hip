@@ -626,7 +626,7 @@ int nsk_row_pass_rows(const nsk_plan* p, int x0, int nx, const void* spec_in, vo
   int rc = check_plan(p);
   if (rc) return rc;
   const int q = row_quantum(p->N);
-  if (x0 < 0 || nx <= 0 || x0 + nx > p->N || x0 % q != 0 || nx % q != 0)
+  if (x0 < 0 || nx <= 0 || x0 > p->N || nx > p->N - x0 || x0 % q != 0 || nx % q != 0)
     return fail(NSK_E_INVAL, "row range [%d, %d) must lie in [0, %d) with both ends multiples of %d", x0,
                 x0 + nx, p->N, q);
   if (scheme != NSK_SCHEME_CENTRAL && scheme != NSK_SCHEME_UPWIND)
@@ -658,7 +658,7 @@ int nsk_column_pass_cols(const nsk_plan* p, int c0, int nc, void* spec, int mode
   int rc = check_plan(p);
   if (rc) return rc;
   const int q = col_quantum(p->N);
-  if (c0 < 0 || nc <= 0 || c0 + nc > p->N / 2 || c0 % q != 0 || nc % q != 0)
+  if (c0 < 0 || nc <= 0 || c0 > p->N / 2 || nc > p->N / 2 - c0 || c0 % q != 0 || nc % q != 0)
     return fail(NSK_E_INVAL, "column range [%d, %d) must lie in [0, %d) with both ends multiples of %d", c0,
                 c0 + nc, p->N / 2, q);
   if (!spec) return fail(NSK_E_INVAL, "nsk_column_pass_cols: NULL buffer");

@@ -182,11 +182,11 @@ def test_slab_range_checks_need_no_gpu():
     p = _lib.make_plan(4096, 2 * math.pi)
     fake = ctypes.c_void_p(4096)
     other = ctypes.c_void_p(8192)
-    for x0, nx in ((0, 0), (-16, 32), (8, 16), (0, 24), (4080, 32)):
+    for x0, nx in ((0, 0), (-16, 32), (8, 16), (0, 24), (4080, 32), (16, 2**31 - 1)):
         rc = lib.nsk_row_pass_rows(ctypes.byref(p), x0, nx, fake, other, fake, 0, 0, 0, None, 0, None)
         with pytest.raises(ValueError, match="row range"):
             _lib.check(rc)
-    for c0, nc in ((0, 0), (1, 2), (0, 3), (2046, 4)):
+    for c0, nc in ((0, 0), (1, 2), (0, 3), (2046, 4), (2, 2**31 - 2)):
         rc = lib.nsk_column_pass_cols(ctypes.byref(p), c0, nc, fake, 0, None)
         with pytest.raises(ValueError, match="column range"):
             _lib.check(rc)
