@@ -137,10 +137,27 @@ def accuracy(solver, u0, u0_dev, theta, N, L, nu, dt, scheme, steps):
             "fp64_oracle_s": round(el, 1)}
 
 
+def _event_us(launch, reps, warm=5):
+    """Average time of `launch` in us: `warm` untimed calls, then `reps`
+    calls between two events on the current stream (the stream the library
+    launches on)."""
+    import torch
+    for _ in range(warm):
+        launch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
 def pass_times(solver, reps):
     """Average launch time (ms) of the column pass and of the fused row pass,
     with torch events on the current stream (the stream the kernels run on)."""
-    import torch
     from ns_solver_jax_differential_distributed_amd import _lib
     lib = _lib.lib()
     p = solver._plan()
@@ -156,17 +173,7 @@ def pass_times(solver, reps):
             else:
                 _lib.check(lib.nsk_row_pass(ctypes.byref(p), s0.data_ptr(), s1.data_ptr(), slots.data_ptr(),
                                             0, scheme, 0, None, 0, st), "row")
-        for _ in range(5):
-            launch()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
-            launch()
-        e1.record()
-        torch.cuda.synchronize()
-        out[name] = e0.elapsed_time(e1) / reps
+        out[name] = _event_us(launch, reps) / 1e3
     return out
 
 
@@ -214,17 +221,7 @@ def dft_gemm_compare(reps, cases=((128, 64), (256, 16), (512, 4))):
     st = _lib.stream_ptr()
 
     def timed(fn):
-        for _ in range(5):
-            fn()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e3
+        return _event_us(fn, reps)
 
     rows = []
     g = torch.Generator(device="cpu").manual_seed(0)
@@ -257,7 +254,6 @@ def slab_shares(solver, reps, parts=(1, 2, 4, 8)):
     pairs [0, N/(2P)) (nsk_row_pass_rows / nsk_column_pass_cols), average
     launch time in us with events on the launch stream, and the share's
     efficiency (whole-field time / P over the share's time)."""
-    import torch
     from ns_solver_jax_differential_distributed_amd import _lib
     lib = _lib.lib()
     p = solver._plan()
@@ -268,17 +264,7 @@ def slab_shares(solver, reps, parts=(1, 2, 4, 8)):
     scheme = _lib.SCHEME[solver.scheme]
 
     def timed(launch):
-        for _ in range(5):
-            launch()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
-            launch()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e3
+        return _event_us(launch, reps)
 
     out = {}
     for P in parts:
@@ -300,7 +286,6 @@ def leg_sweep(steps=200, warmup=40):
     4096} x {central, upwind} x nu in {1e-4, 1e-3}, one field each, Taylor-
     Green + perturbation u0, dt from stable_dt for the run; cell-updates/s
     over `steps` graph-replayed steps and the two passes' launch times."""
-    import numpy as np
     import torch
     from ns_solver_jax_differential_distributed_amd import NS_solver as S
     from ns_solver_jax_differential_distributed_amd import workloads as W
@@ -385,9 +370,7 @@ def leg_serving(n=300, rate=200.0):
     """Poisson stream (200 req/s) of 70% rollout / 30% rollout+gradient
     requests, N from {128..2048}, steps in [50, 500]: p50/p99 latency at that
     rate, and solves/s at saturation against plain per-request execution."""
-    import numpy as np
     import torch
-    from ns_solver_jax_differential_distributed_amd import NS_solver as S
     from ns_solver_jax_differential_distributed_amd import engine as E
     from ns_solver_jax_differential_distributed_amd import serving as SV
     from ns_solver_jax_differential_distributed_amd import workloads as W
@@ -452,7 +435,6 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
     from ns_solver_jax_differential_distributed_amd import NS_solver as S
