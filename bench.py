@@ -421,6 +421,11 @@ def leg_serving(n=300, rate=200.0):
 
 def main():
     args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -430,11 +435,6 @@ def main():
     if args.profile_gradient:
         profile_gradient(args.N, args.steps, args.nu, args.scheme)
         return
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
-        sys.exit(2)
     import torch
 
     from ns_solver_jax_differential_distributed_amd import NS_solver as S

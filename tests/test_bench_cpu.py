@@ -1,0 +1,32 @@
+"""bench.py's command-line contract, without a GPU: one GPU only (any
+other --gpus or a WORLD_SIZE above 1 exits with status 2 and says so,
+before anything touches a GPU), and the documented options parse."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run(*args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True,
+                          text=True, env=e, timeout=120)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    for args, env in ((("--gpus", "2"), None), (("--gpus", "8", "--dft-gemm"), None),
+                      (("--profile-gradient",), {"WORLD_SIZE": "4"}), ((), {"WORLD_SIZE": "2"})):
+        r = run(*args, env=env)
+        assert r.returncode == 2, (args, r.returncode, r.stderr)
+        assert "one GPU only" in r.stderr
+        assert r.stdout == ""
+
+
+def test_options_parse():
+    r = run("--help")
+    assert r.returncode == 0
+    for opt in ("--steps", "--warmup", "--gpus", "--N", "--scheme", "--no-extras", "--no-cpu-baseline",
+                "--calibrate-copy", "--dft-gemm", "--profile-gradient"):
+        assert opt in r.stdout, opt
