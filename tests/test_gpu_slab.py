@@ -27,19 +27,19 @@ def setup(N, B, seed):
     return p, spec, slots, npart
 
 
-def row_full(p, src, slots, npart, B):
+def row_full(p, src, slots, npart, B, scheme=0):
     _lib = L()
     out = torch.full_like(src, float("nan"))
     jp = torch.full((B, npart), float("nan"), dtype=torch.float64, device="cuda")
     _lib.check(_lib.lib().nsk_row_pass(ctypes.byref(p), src.data_ptr(), out.data_ptr(), slots.data_ptr(),
-                                       0, 0, 0, jp.data_ptr(), 1, _lib.stream_ptr()), "row")
+                                       0, scheme, 0, jp.data_ptr(), 1, _lib.stream_ptr()), "row")
     return out, jp
 
 
-def row_range(p, src, out, slots, jp, x0, nx):
+def row_range(p, src, out, slots, jp, x0, nx, scheme=0):
     _lib = L()
     _lib.check(_lib.lib().nsk_row_pass_rows(ctypes.byref(p), x0, nx, src.data_ptr(), out.data_ptr(),
-                                            slots.data_ptr(), 0, 0, 0, jp.data_ptr(), 1,
+                                            slots.data_ptr(), 0, scheme, 0, jp.data_ptr(), 1,
                                             _lib.stream_ptr()), "row range")
 
 
@@ -59,16 +59,17 @@ def same(a, b):
     return torch.equal(a.view(torch.int32), b.view(torch.int32))
 
 
-@pytest.mark.parametrize("N,B,parts,r", [(4096, 1, 8, 3), (2048, 1, 8, 7), (1024, 2, 4, 0), (256, 3, 2, 1),
-                                             (128, 2, 2, 1)])
-def test_one_share_equals_the_whole_field_passes_there(N, B, parts, r):
+@pytest.mark.parametrize("N,B,parts,r,scheme", [(4096, 1, 8, 3, 0), (2048, 1, 8, 7, 0), (1024, 2, 4, 0, 0),
+                                                    (1024, 2, 4, 2, 1), (256, 3, 2, 1, 0), (128, 2, 2, 1, 1)])
+def test_one_share_equals_the_whole_field_passes_there(N, B, parts, r, scheme):
+    """scheme 0 central, 1 upwind advection."""
     p, spec, slots, npart = setup(N, B, seed=N + r)
-    full, jfull = row_full(p, spec, slots, npart, B)
+    full, jfull = row_full(p, spec, slots, npart, B, scheme)
     nx = N // parts
     x0 = r * nx
     out = torch.full_like(spec, float("nan"))
     jp = torch.full((B, npart), float("nan"), dtype=torch.float64, device="cuda")
-    row_range(p, spec, out, slots, jp, x0, nx)
+    row_range(p, spec, out, slots, jp, x0, nx, scheme)
     torch.cuda.synchronize()
     assert same(out[:, x0:x0 + nx], full[:, x0:x0 + nx])
     assert torch.isnan(out[:, :x0]).all() and torch.isnan(out[:, x0 + nx:]).all()
