@@ -159,6 +159,31 @@ def test_gradient_without_recomputation_is_bitwise_equal(scheme):
 
 
 @pytest.mark.parametrize("scheme", ["central", "upwind"])
+def test_gradient_at_the_largest_grid_is_independent_of_checkpointing(scheme):
+    """N=4096 (the largest supported grid), 12 steps: every state kept and
+    the reverse sweep replayed from graphs, every state kept and launched
+    directly, and segments of 5 recomputed from checkpoints give the same
+    bits for J, u_final, dJ/dtheta and dJ/du0."""
+    N, L, nu, steps = 4096, 2 * math.pi, 1e-4, 12
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    u0 = W.initial_velocity(N, seed=1, kmax=8)
+    dt = W.cfl_dt(u0, L, 0.3, nu, stable=True, steps=steps)
+    u0t = torch.from_numpy(u0)
+    theta = [1.1, 0.7, 0.9]
+    outs = []
+    for k, graph in ((1, True), (1, False), (5, False)):
+        s = solver(N, L, nu, dt, steps, scheme, ckpt_every=k, use_graph=graph)
+        outs.append([t.cpu() for t in s.rollout_and_grad(u0t, theta)])
+        del s
+        torch.cuda.empty_cache()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.equal(a, b)
+    J, uf, g, du0 = outs[0]
+    assert torch.isfinite(uf).all() and torch.isfinite(du0).all() and float(g[2]) == 0.0
+
+
+@pytest.mark.parametrize("scheme", ["central", "upwind"])
 def test_graph_replayed_gradient_equals_eager(scheme):
     """75 steps with every state kept: the forward in graph-replayed chunks and
     the reverse sweep as two 32-step chunks replayed from one captured graph
