@@ -507,9 +507,9 @@ class SingleCoreNSSolver:
     Extra keyword arguments (not in the reference):
       objective: "work" (reference J, NS_solver.py:746) or "mean_ke"
       ckpt_every: checkpoint interval of the adjoint's recomputation, or
-        "auto" (default): every state kept when the trajectory fits 8 GiB of
-        HBM (no recomputation, as the reference's reverse mode keeps every
-        scan state), else about sqrt(steps)
+        "auto" (default): every state kept when the trajectory fits half of
+        the free device memory (no recomputation, as the reference's reverse
+        mode keeps every scan state), else about sqrt(steps)
       use_graph: replay the step sequence through HIP graphs
     """
 

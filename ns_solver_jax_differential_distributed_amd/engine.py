@@ -183,18 +183,32 @@ class RolloutEngine:
             "rollout_adjoint")
         return dth, du0
 
+    # "auto" keeps every state when the trajectory fits this share of the
+    # device memory free at the call (8 GiB when that cannot be queried)
+    AUTO_CKPT_FRACTION = 0.5
     AUTO_CKPT_BYTES = 8 << 30
     JCAP_QUANTUM = 1024
+
+    def auto_ckpt_bytes(self) -> int:
+        """Checkpoint memory "auto" may use: AUTO_CKPT_FRACTION of the free
+        device memory (half of an idle MI355X's 288 GB holds an N=4096 field
+        for 1000 steps), or AUTO_CKPT_BYTES when it cannot be queried."""
+        try:
+            free, _ = torch.cuda.mem_get_info()
+        except (RuntimeError, AssertionError):
+            return self.AUTO_CKPT_BYTES
+        return int(free * self.AUTO_CKPT_FRACTION)
 
     def resolve_ckpt(self, ckpt_every, steps: int) -> int:
         """The checkpoint interval for ``ckpt_every``: an int as given, or for
         "auto" (also None or 0) 1 - every state kept, the reverse sweep
-        recomputes nothing - when all slots' trajectories fit AUTO_CKPT_BYTES,
-        else ~sqrt(steps), which minimises checkpoints + segment."""
+        recomputes nothing - when all slots' trajectories fit auto_ckpt_bytes(),
+        else ~sqrt(steps), which minimises checkpoints + segment.  The
+        interval changes the memory and time of a gradient, not its bits."""
         steps = max(int(steps), 1)
         if ckpt_every not in (None, 0, "auto"):
             return max(1, min(int(ckpt_every), steps))
-        if self.batch * steps * 8 * self.N * self.N <= self.AUTO_CKPT_BYTES:
+        if self.batch * steps * 8 * self.N * self.N <= self.auto_ckpt_bytes():
             return 1
         return max(1, int(math.isqrt(steps)))
 

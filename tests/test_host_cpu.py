@@ -195,6 +195,29 @@ def test_slab_range_checks_need_no_gpu():
         _lib.check(rc)
 
 
+def test_auto_checkpoint_interval(monkeypatch):
+    """ckpt_every="auto": every state kept when the trajectory fits half the
+    free device memory, else about sqrt(steps); 8 GiB when the free memory
+    cannot be queried."""
+    import torch
+    from ns_solver_jax_differential_distributed_amd import engine as E
+    x = object.__new__(E.RolloutEngine)
+    x.batch, x.N = 1, 4096                      # 128 MiB per state
+
+    def no_gpu():
+        raise RuntimeError("no device")
+    monkeypatch.setattr(torch.cuda, "mem_get_info", no_gpu)
+    assert x.auto_ckpt_bytes() == 8 << 30
+    assert x.resolve_ckpt("auto", 64) == 1      # 8 GiB exactly
+    assert x.resolve_ckpt(None, 65) == 8        # isqrt(65)
+    assert x.resolve_ckpt(0, 1000) == 31
+    assert x.resolve_ckpt(7, 1000) == 7 and x.resolve_ckpt(50, 20) == 20
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda: (280 << 30, 288 << 30))
+    assert x.auto_ckpt_bytes() == 140 << 30
+    assert x.resolve_ckpt("auto", 1000) == 1    # 125 GiB of checkpoints
+    assert x.resolve_ckpt("auto", 1200) == 34   # 150 GiB do not fit
+
+
 def test_slot_record_layout():
     import torch
     rec = _lib.make_slots(0.01, 1e-3, 0.9, 0.4, 5, 100, batch=2, device=torch.device("cpu"))
