@@ -77,6 +77,19 @@ class SolveResult:
     def latency(self) -> float:
         return self.finished - self.arrival
 
+    @property
+    def finite(self) -> bool:
+        """False when the solve blew up: J sums u_n . f_n (or |u_n|^2) over
+        every step's state in FP64, so a NaN or an overflow anywhere in the
+        trajectory before the last step reaches it (and the gradient, when
+        there is one).  Costs nothing: J is on the host already.  Slots are
+        independent, so one request blowing up leaves the others' bits as
+        they are."""
+        ok = math.isfinite(self.J)
+        if self.dtheta is not None:
+            ok = ok and bool(np.isfinite(self.dtheta).all())
+        return ok
+
 
 class HBMBudget:
     def __init__(self, nbytes: int):

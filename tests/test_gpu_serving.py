@@ -108,3 +108,36 @@ def test_hbm_budget_bounds_admission():
     assert len(out) == 6
     assert srv.buckets[64].cap == 2          # halved until it fit the budget
     assert srv.budget.used <= srv.budget.limit
+
+
+def test_a_request_that_blows_up_is_flagged_and_leaves_the_others_untouched():
+    """One request with a NaN in u0 and one whose dt is far beyond the
+    explicit-Euler limit, among ordinary ones and a gradient request in the
+    same buckets: the two come back with ``finite`` False (J and, for the
+    gradient, dJ/dtheta non-finite), every other result equals the same mix
+    without them bit for bit, and nothing hangs or faults."""
+    _, serving, _ = M()
+    good = [_req(serving, N, st, seed=40 + i, grad=(i == 2)) for i, (N, st) in
+            enumerate([(64, 30), (64, 44), (64, 26), (128, 21)])]
+    nan_req = _req(serving, 64, 35, seed=50)
+    u = nan_req.u0.clone()
+    u[0, 7, 9] = float("nan")
+    nan_req.u0 = u
+    big_dt = _req(serving, 64, 40, seed=51, grad=True)
+    big_dt.dt = 50.0 * big_dt.dt
+
+    def run(reqs):
+        srv = serving.NSServer(chunk=8, max_steps=64, capacity={64: 4, 128: 2})
+        return {r.rid: r for r in srv.run(reqs, realtime=False)}
+
+    clean = run([_req(serving, N, st, seed=40 + i, grad=(i == 2)) for i, (N, st) in
+                 enumerate([(64, 30), (64, 44), (64, 26), (128, 21)])])
+    mixed = run(good[:2] + [nan_req] + good[2:] + [big_dt])
+    assert not mixed[nan_req.rid].finite and not mixed[big_dt.rid].finite
+    assert not math.isfinite(mixed[nan_req.rid].J)
+    for a, b in zip(sorted(clean.values(), key=lambda r: r.rid),
+                    [mixed[r.rid] for r in good]):
+        assert b.finite
+        assert b.J == a.J and torch.equal(b.u_final, a.u_final)
+        if a.dtheta is not None:
+            assert np.array_equal(a.dtheta, b.dtheta) and torch.equal(a.du0, b.du0)

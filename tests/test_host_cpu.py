@@ -218,6 +218,17 @@ def test_auto_checkpoint_interval(monkeypatch):
     assert x.resolve_ckpt("auto", 1200) == 34   # 150 GiB do not fit
 
 
+def test_served_result_finite_flag():
+    import numpy as np
+    from ns_solver_jax_differential_distributed_amd import serving as SV
+
+    def res(J, dtheta=None):
+        return SV.SolveResult(0, 64, 10, J, None, dtheta, None, 0.0, 0.0, 1.0)
+    assert res(1.5).finite and res(1.5, np.array([0.1, -2.0])).finite
+    assert not res(float("nan")).finite and not res(float("inf")).finite
+    assert not res(1.5, np.array([0.1, float("nan")])).finite
+
+
 def test_slot_record_layout():
     import torch
     rec = _lib.make_slots(0.01, 1e-3, 0.9, 0.4, 5, 100, batch=2, device=torch.device("cpu"))
