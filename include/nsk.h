@@ -169,7 +169,7 @@ int nsk_col_quantum(int N);
  * there); slots[b].step must be 0 on entry and is advanced by `steps`.  J_out[b] (double, device)
  * receives the objective.  spec0/spec1: two spectrum buffers.
  * jpart: nsk_partials_per_slot(p, jcap) doubles per slot, jcap >= steps (0
- * means steps); the cached step graphs depend on jcap and the buffer
+ * means steps), scratch (its contents on return are unspecified); the cached step graphs depend on jcap and the buffer
  * addresses, not on `steps`, so callers that keep one jpart of a fixed jcap
  * replay the same graphs for rollouts of any length up to it.
  * ckpt (nullable): stores u_n for n % ckpt_every == 0, n < steps, as
@@ -191,7 +191,7 @@ int nsk_rollout(const nsk_plan *p, const float *u0, float *u_final,
  * slot; seg: ckpt_every fields of (2,N,N) per slot (unused, may be NULL, when
  * ckpt_every == 1: the reverse sweep then reads every u_n from ckpt and
  * recomputes nothing); gpart:
- * 2*nsk_partials_per_slot(p, steps) doubles per slot.  slots[b].step must
+ * 2*nsk_partials_per_slot(p, steps) doubles per slot (scratch).  slots[b].step must
  * be 0 on entry and is `steps` on return.  use_graph != 0 replays the
  * forward chunks and, with ckpt_every == 1, the reverse sweep in chunks of
  * 32 steps from HIP graphs (captured on first use, cached per buffer set). */

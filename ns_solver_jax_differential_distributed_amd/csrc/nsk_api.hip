@@ -207,20 +207,15 @@ __global__ void k_stencil(StencilArgs a) {
 }
 
 // out[b*ncomp + c] = sum_{n < nsteps_b, band} part[b][n][band][c], in a
-// fixed order (thread-strided partial sums, then a fixed tree).  Each
-// thread issues kFinU loads before adding them in the same order as one at
-// a time, so the sum keeps its bits and the loads overlap.
-constexpr int kFinT = 1024, kFinU = 32;
-__global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap, int nb, int ncomp,
-                                                    const nsk_slot* slots, double* out) {
-  __shared__ double red[kFinT];
-  const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
-  int nsteps = slots[b].nsteps;
-  if (nsteps > pcap) nsteps = pcap;
-  const size_t total = (size_t)nsteps * nb;
-  const double* base = part + (size_t)b * pcap * nb * ncomp;
+// fixed order: kFinT chains (chain i adds elements i, i + kFinT, ... in
+// order), then a fixed tree over the chain sums.  Each chain issues kFinU
+// loads before adding them in the same order as one at a time, so the sum
+// keeps its bits and the loads overlap.
+constexpr int kFinT = 1024, kFinU = 32, kFinChain = 64;
+
+__device__ __forceinline__ double fin_chain(const double* base, size_t total, int ncomp, int c, int i) {
   double s = 0.0;
-  size_t k = tid;
+  size_t k = i;
   for (; k + (size_t)(kFinU - 1) * kFinT < total; k += (size_t)kFinU * kFinT) {
     double v[kFinU];
 #pragma unroll
@@ -229,12 +224,57 @@ __global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap
     for (int j = 0; j < kFinU; ++j) s += v[j];
   }
   for (; k < total; k += kFinT) s += base[k * ncomp + c];
-  red[tid] = s;
+  return s;
+}
+
+__device__ __forceinline__ size_t fin_total(const nsk_slot* slots, int b, int pcap, int nb) {
+  int nsteps = slots[b].nsteps;
+  if (nsteps > pcap) nsteps = pcap;
+  return (size_t)nsteps * nb;
+}
+
+// red[0..kFinT) -> red[0] by the fixed tree; every thread of the block calls it.
+__device__ __forceinline__ void fin_tree(double* red, int tid) {
   __syncthreads();
   for (int w = kFinT / 2; w > 0; w >>= 1) {
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
+}
+
+// One workgroup per (slot, component); the partials are only read.
+__global__ void __launch_bounds__(kFinT) k_finalize(const double* part, int pcap, int nb, int ncomp,
+                                                    const nsk_slot* slots, double* out) {
+  __shared__ double red[kFinT];
+  const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const size_t total = fin_total(slots, b, pcap, nb);
+  red[tid] = fin_chain(part + (size_t)b * pcap * nb * ncomp, total, ncomp, c, tid);
+  fin_tree(red, tid);
+  if (tid == 0) out[(size_t)b * ncomp + c] = red[0];
+}
+
+// The same sum with the chains spread over kFinT / kFinChain workgroups per
+// (slot, component), for partials that are scratch to the caller: chain i
+// stores its sum over element i, which no other chain reads (k_fin_tree then
+// runs the same tree over elements 0 .. kFinT-1).  Same bits as k_finalize,
+// a fraction of its time once the partials exceed what one CU streams fast.
+__global__ void __launch_bounds__(kFinChain) k_fin_chains(double* part, int pcap, int nb, int ncomp,
+                                                         const nsk_slot* slots) {
+  const int b = blockIdx.y, c = blockIdx.z, i = blockIdx.x * kFinChain + threadIdx.x;
+  const size_t total = fin_total(slots, b, pcap, nb);
+  if ((size_t)i >= total) return;
+  double* base = part + (size_t)b * pcap * nb * ncomp;
+  const double s = fin_chain(base, total, ncomp, c, i);
+  base[(size_t)i * ncomp + c] = s;
+}
+
+__global__ void __launch_bounds__(kFinT) k_fin_tree(const double* part, int pcap, int nb, int ncomp,
+                                                    const nsk_slot* slots, double* out) {
+  __shared__ double red[kFinT];
+  const int b = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const size_t total = fin_total(slots, b, pcap, nb);
+  red[tid] = ((size_t)tid < total) ? part[(size_t)b * pcap * nb * ncomp + (size_t)tid * ncomp + c] : 0.0;
+  fin_tree(red, tid);
   if (tid == 0) out[(size_t)b * ncomp + c] = red[0];
 }
 
@@ -278,6 +318,15 @@ hipError_t finalize(const nsk_plan* p, const nsk_slot* slots, const double* part
                     int ncomp, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(p->batch, ncomp), dim3(kFinT), 0, st, part, pcap,
                      row_parts(p->N), ncomp, slots, out);
+  return hipGetLastError();
+}
+// Same result; overwrites the first kFinT elements of each slot's partials.
+hipError_t finalize_scratch(const nsk_plan* p, const nsk_slot* slots, double* part, int pcap,
+                            int ncomp, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_fin_chains, dim3(kFinT / kFinChain, p->batch, ncomp), dim3(kFinChain), 0, st, part,
+                     pcap, row_parts(p->N), ncomp, slots);
+  hipLaunchKernelGGL(k_fin_tree, dim3(p->batch, ncomp), dim3(kFinT), 0, st, part, pcap, row_parts(p->N),
+                     ncomp, slots, out);
   return hipGetLastError();
 }
 
@@ -706,7 +755,7 @@ static int rollout_impl(const nsk_plan* p, const float* u0, float* u_final, nsk_
   int rc = run_steps(p, slots, steps, scheme, objective, &cur, &nxt, u_final, jpart, jcap, ckpt,
                      ckpt_every, ckpt_cap, use_graph, st);
   if (rc) return rc;
-  if (J_out) HIP_TRY(finalize(p, slots, jpart, jcap, 1, J_out, st));
+  if (J_out) HIP_TRY(finalize_scratch(p, slots, jpart, jcap, 1, J_out, st));
   return NSK_OK;
 }
 
@@ -900,7 +949,7 @@ static int adjoint_impl(const nsk_plan* p, nsk_slot* slots, int steps, int schem
     w.dst_bstride = NN2;
     HIP_TRY(launch_row(p->N, RK_SPEC_TO_REAL, 0, w, p->batch, st));
   }
-  if (dtheta_out) HIP_TRY(finalize(p, slots, gpart, steps, 2, dtheta_out, st));
+  if (dtheta_out) HIP_TRY(finalize_scratch(p, slots, gpart, steps, 2, dtheta_out, st));
   return NSK_OK;
 }
 
