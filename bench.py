@@ -475,7 +475,7 @@ def main():
                 "pass_us": {k: round(v * 1e3, 2) for k, v in pt.items()}}
     step_us = el / args.steps * 1e6
     line = {
-        "metric": "grid-cell updates/s at N=4096 fp32",
+        "metric": "grid-cell updates/s at N=4096 fp32; served rollout+grad solves/s & p99 latency",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": step_us / 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
