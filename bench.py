@@ -63,6 +63,8 @@ def parse():
                     help="only run 10 device copies of 256 MiB (scale for HBM byte counters)")
     ap.add_argument("--dft-gemm", action="store_true",
                     help="only compare the column pass at small N with the same transforms as FP32 GEMMs")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only run the serving leg (for a kernel trace of the served mix)")
     ap.add_argument("--profile-gradient", action="store_true",
                     help="only run rollout + gradient (every state checkpointed) of the workload at --N "
                          "for --steps steps, three times, for a kernel trace of the gradient path")
@@ -434,6 +436,9 @@ def main():
         return
     if args.profile_gradient:
         profile_gradient(args.N, args.steps, args.nu, args.scheme)
+        return
+    if args.serving_only:
+        print(json.dumps({"serving": leg_serving()}), flush=True)
         return
     import torch
 
