@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import math
 import os
@@ -383,35 +384,54 @@ def leg_serving(n=300, rate=200.0):
         for r in reqs:
             r.u0 = torch.from_numpy(W.initial_velocity(r.N, seed=r.seed, kmax=min(8, r.N // 2 - 1))).cuda()
         return reqs
+    def release():
+        # earlier legs' and servers' blocks cached by torch would otherwise
+        # push the next run into allocator retries (free-all-and-retry, which
+        # synchronizes the device)
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    release()
     reqs = make()
-    warm = SV.NSServer(max_steps=512)
-    warm.run([SV.SolveRequest(N=r.N, steps=20, theta=r.theta, nu=r.nu, dt=r.dt, u0=r.u0,
-                              want_grad=r.want_grad) for r in reqs[:12]], realtime=False)
-    srv = SV.NSServer(max_steps=512)
-    res = srv.run(reqs, realtime=True)
+
+    def warm_set():
+        # one rollout and one gradient request of 20 steps per grid size, run
+        # on the very server or engines that are then timed: both paths start
+        # with their kernels loaded, graphs captured and buffers allocated
+        first = {}
+        for r in reqs:
+            first.setdefault(r.N, r)
+        return [SV.SolveRequest(N=r.N, steps=20, theta=r.theta, nu=r.nu, dt=r.dt, u0=r.u0, want_grad=g)
+                for r in first.values() for g in (False, True)]
+    with SV.NSServer(max_steps=512) as srv:
+        srv.run(warm_set(), realtime=False)
+        torch.cuda.synchronize()
+        res = srv.run(reqs, realtime=True)
     live = SV.latency_stats(res)
-    srv2 = SV.NSServer(max_steps=512)
-    reqs2 = make()
-    torch.cuda.synchronize()
-    t_sat, res2 = _timed(lambda: srv2.run(reqs2, realtime=False))
+    del res
+    release()
+    with SV.NSServer(max_steps=512) as srv2:
+        srv2.run(warm_set(), realtime=False)
+        reqs2 = make()
+        torch.cuda.synchronize()
+        t_sat, res2 = _timed(lambda: srv2.run(reqs2, realtime=False))
+    release()
     # plain per-request path: one solver call per request, back to back
     engines = {}
 
+    def plain_one(r):
+        eng = engines.get(r.N)
+        if eng is None:
+            eng = engines[r.N] = E.RolloutEngine(r.N, L, 1)
+        if r.want_grad:
+            return eng.rollout_and_grad(r.u0, r.nu, r.dt, r.theta, r.steps, "auto")
+        return eng.rollout(r.u0, r.nu, r.dt, r.theta, r.steps)
+
     def plain():
-        out = []
-        for r in reqs2:
-            eng = engines.get(r.N)
-            if eng is None:
-                eng = engines[r.N] = E.RolloutEngine(r.N, L, 1)
-            if r.want_grad:
-                out.append(eng.rollout_and_grad(r.u0, r.nu, r.dt, r.theta, r.steps, "auto"))
-            else:
-                out.append(eng.rollout(r.u0, r.nu, r.dt, r.theta, r.steps))
-        return out
-    plain_small = [r for r in reqs2[:8]]
-    for r in plain_small:
-        eng = engines.get(r.N) or engines.setdefault(r.N, E.RolloutEngine(r.N, L, 1))
-        eng.rollout(r.u0, r.nu, r.dt, r.theta, 20)
+        return [plain_one(r) for r in reqs2]
+    for r in warm_set():
+        plain_one(r)
+    torch.cuda.synchronize()
     t_plain, outs = _timed(plain)
     byid = {r.rid: r for r in res2}
     same = all(float(byid[r.rid].J) == float(o[0][0]) for r, o in zip(reqs2, outs))

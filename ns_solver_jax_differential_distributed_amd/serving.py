@@ -326,7 +326,12 @@ class NSServer:
     GPU time (estimated from N and the bucket's capacity); streams:
     "shared" (one stream for the buckets up to N = SMALL_N, one for the
     larger ones, one for gradient groups) or "per_bucket"; hbm_budget:
-    bytes the server may allocate.
+    bytes the server may allocate; keep_states_fraction: a gradient group
+    keeps every state (no recomputation in its reverse sweep) when its
+    trajectory fits this fraction of hbm_budget, else checkpoints every
+    grad_ckpt_every steps.  The gradient is bit-identical either way; keeping
+    every state made the 300-request mix 13 % faster than checkpointing
+    every group (0.5 against 0.125 of a 96 GiB budget, measured).
 
     The device runs a few hardware queues, and streams beyond their number
     share them in order: a stream of small kernels placed behind a long
@@ -334,7 +339,9 @@ class NSServer:
     Shared streams keep the small buckets on a queue of their own, and
     chunks sized by GPU time keep any one launch short."""
 
-    DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 48, 1024: 16, 2048: 6, 4096: 2}
+    # N = 1024 / 2048 at 24 / 8 slots: 7 % more saturated throughput on a
+    # 300-request mix than 16 / 6 (measured); idle slots exit at once
+    DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 48, 1024: 24, 2048: 8, 4096: 2}
     SMALL_N = 256
     # steady cell-updates/s of a full launch and the floor of one step's two
     # launches, for sizing chunks (measured on MI355X)
@@ -345,9 +352,12 @@ class NSServer:
                  objective: str = "work", chunk: Optional[int] = 16, max_steps: int = 512,
                  capacity: Optional[Dict[int, int]] = None, hbm_budget: int = 96 << 30,
                  grad_group: int = 64, grad_ckpt_every: int = 25, use_graph: bool = True,
-                 streams: str = "per_bucket", chunk_us: float = 500.0):
+                 streams: str = "per_bucket", chunk_us: float = 500.0,
+                 keep_states_fraction: float = 0.5):
         if chunk is not None and (chunk < 2 or chunk % 2):
             raise ValueError("chunk must be an even number >= 2")
+        if not 0.0 <= keep_states_fraction <= 1.0:
+            raise ValueError("keep_states_fraction must be in [0, 1]")
         if streams not in ("shared", "per_bucket"):
             raise ValueError("streams must be 'shared' or 'per_bucket'")
         if scheme not in _lib.SCHEME or objective not in _lib.OBJECTIVE:
@@ -360,6 +370,7 @@ class NSServer:
         self.capacity = dict(self.DEFAULT_CAPACITY if capacity is None else capacity)
         self.budget = HBMBudget(hbm_budget)
         self.grad_group, self.grad_ckpt_every = int(grad_group), int(grad_ckpt_every)
+        self.keep_states_fraction = float(keep_states_fraction)
         self.use_graph = bool(use_graph)
         self.worker = ThreadPoolExecutor(max_workers=1, thread_name_prefix="nsk-grad")
         self.buckets: Dict[int, _Bucket] = {}
@@ -368,6 +379,7 @@ class NSServer:
         self.gqueue: Dict[int, deque] = {}
         self.results: List[SolveResult] = []
         self._rid = 0
+        self.closed = False
         self.t0 = time.perf_counter()
 
     def now(self) -> float:
@@ -394,6 +406,8 @@ class NSServer:
 
     # ---------------------------------------------------------------- requests
     def submit(self, req: SolveRequest) -> int:
+        if self.closed:
+            raise RuntimeError("NSServer is closed")
         check_n(req.N)
         if req.steps < 1 or req.steps > self.max_steps:
             raise ValueError(f"steps must be in [1, {self.max_steps}], got {req.steps}")
@@ -458,9 +472,9 @@ class NSServer:
             grp = [q.popleft() for _ in range(min(self.grad_group, len(q)))]
             smax = max(r.steps for r in grp)
             # every state checkpointed (no recomputation in the reverse sweep)
-            # when the group's trajectory fits half the budget
+            # when the group's trajectory fits keep_states_fraction of the budget
             k = self.grad_ckpt_every
-            if _GradLane.group_bytes(N, len(grp), smax, 1) <= self.budget.limit // 2:
+            if _GradLane.group_bytes(N, len(grp), smax, 1) <= self.budget.limit * self.keep_states_fraction:
                 k = 1
             need = _GradLane.group_bytes(N, len(grp), smax, k)
             if not self.budget.take(need):
@@ -487,6 +501,28 @@ class NSServer:
             self.budget.give(lane.harvest(self.now, self.results, False))
         if not launched:
             time.sleep(5e-5)
+
+    def close(self) -> None:
+        """Wait for the work in flight, stop the gradient worker and drop the
+        buckets, lanes and their captured graphs, so their device memory
+        returns to torch's allocator.  Results already returned stay valid;
+        the server accepts no further requests."""
+        for lane in self.lanes.values():
+            lane.wait()
+        torch.cuda.synchronize()
+        self.worker.shutdown(wait=True)
+        self.buckets.clear()
+        self.lanes.clear()
+        self._shared.clear()
+        self.queue.clear()
+        self.gqueue.clear()
+        self.closed = True
+
+    def __enter__(self) -> "NSServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
     def run(self, requests: Sequence[SolveRequest], realtime: bool = True) -> List[SolveResult]:
         """Serve requests (their ``arrival`` fields are offsets from the start

@@ -141,3 +141,24 @@ def test_a_request_that_blows_up_is_flagged_and_leaves_the_others_untouched():
         assert b.J == a.J and torch.equal(b.u_final, a.u_final)
         if a.dtheta is not None:
             assert np.array_equal(a.dtheta, b.dtheta) and torch.equal(a.du0, b.du0)
+
+
+def test_checkpoint_policy_leaves_gradients_bit_identical_and_close_releases():
+    """A gradient group keeps every state or checkpoints every k steps
+    depending on keep_states_fraction; the reverse sweep recomputes the same
+    float32 states either way, so results are bit-identical.  close() keeps
+    returned results valid and refuses further requests."""
+    _, serving, _ = M()
+    out = []
+    for frac in (0.0, 1.0):
+        reqs = [_req(serving, 64, st, seed=70 + i, grad=True) for i, st in enumerate([41, 23, 64])]
+        with serving.NSServer(chunk=8, max_steps=64, grad_group=4, grad_ckpt_every=6,
+                              keep_states_fraction=frac) as srv:
+            res = sorted(srv.run(reqs, realtime=False), key=lambda r: r.rid)
+        assert srv.closed and not srv.buckets and not srv.lanes
+        with pytest.raises(RuntimeError):
+            srv.submit(_req(serving, 64, 5, seed=1))
+        out.append(res)
+    for a, b in zip(*out):
+        assert a.J == b.J and np.array_equal(a.dtheta, b.dtheta)
+        assert torch.equal(a.u_final, b.u_final) and torch.equal(a.du0, b.du0)

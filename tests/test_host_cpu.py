@@ -229,6 +229,17 @@ def test_served_result_finite_flag():
     assert not res(1.5, np.array([0.1, float("nan")])).finite
 
 
+def test_server_arguments_are_validated():
+    from ns_solver_jax_differential_distributed_amd import serving as SV
+    for kw in ({"keep_states_fraction": -0.1}, {"keep_states_fraction": 1.5}, {"chunk": 3},
+               {"streams": "one"}, {"scheme": "weno"}):
+        with pytest.raises(ValueError):
+            SV.NSServer(**kw)
+    srv = SV.NSServer(keep_states_fraction=0.25)
+    assert srv.keep_states_fraction == 0.25 and not srv.closed
+    srv.worker.shutdown(wait=True)
+
+
 def test_slot_record_layout():
     import torch
     rec = _lib.make_slots(0.01, 1e-3, 0.9, 0.4, 5, 100, batch=2, device=torch.device("cpu"))
