@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--profile-gradient", action="store_true",
                     help="only run rollout + gradient (every state checkpointed) of the workload at --N "
                          "for --steps steps, three times, for a kernel trace of the gradient path")
+    ap.add_argument("--profile-ops", action="store_true",
+                    help="only run each operator entry point (stencils, advection, forcing, Poisson, "
+                         "projection, projector step) at --N, --reps times, for a kernel trace")
     return ap.parse_args()
 
 
@@ -207,6 +210,42 @@ def profile_gradient(N, steps, nu, scheme, reps=3):
     t, _ = _timed(lambda: [solver.rollout_and_grad(u0_dev, theta) for _ in range(reps)])
     print(json.dumps({"profile_gradient": {"N": N, "steps": steps, "scheme": scheme, "ckpt_every": 1,
                                            "us_per_step": t / reps / steps * 1e6}}), flush=True)
+
+
+def profile_ops(N, reps):
+    """The operator entry points one by one at N (the reference's
+    DifferentialOps, advection schemes, forcing, Poisson solve, projection
+    and fused projector step, NS_solver.py:206-265,468-527,617-697), each
+    `reps` times after two warm-up calls, for a kernel trace; prints the
+    average time per call from events on the current stream."""
+    import torch
+    from ns_solver_jax_differential_distributed_amd import NS_solver as S
+    from ns_solver_jax_differential_distributed_amd import _lib
+    from ns_solver_jax_differential_distributed_amd import workloads as W
+    L = 2 * math.pi
+    dx = L / N
+    ops = S.PeriodicDifferentialOps(dx)
+    u = torch.from_numpy(W.initial_velocity(N, seed=0, kmax=8)).cuda()
+    f = u[0].clone()       # a field of its own, not an alias of u_x
+    pois = S.SpectralPoissonSolver(N, N, dx)
+    forcing = S.GaussianForcingField(S.make_grid(N, L), S.Phys(1e-4, 0.5, 0.1, 1.0))
+    proj = S.ChorinProjector(ops, pois, S.CentralAdvection(), 1e-4, 1e-4)
+    fu = forcing([1.0, 0.5], 0.0)
+    calls = {
+        "grad": lambda: ops.grad(f),
+        "laplacian": lambda: ops.laplacian(f),
+        "divergence": lambda: ops.divergence(u),
+        "advect_central": lambda: ops.advect(f, u[0], u[1], _lib.SCHEME["central"]),
+        "advect_upwind": lambda: ops.upwind_advect(f, u[0], u[1]),
+        "forcing": lambda: forcing([1.0, 0.5], 0.25),
+        "poisson": lambda: pois(f),
+        "project": lambda: pois.project(u),
+        "projector_step": lambda: proj.step(u, fu),
+    }
+    out = {}
+    for name, fn in calls.items():
+        out[name] = round(_event_us(fn, reps, warm=2), 2)
+    print(json.dumps({"profile_ops": {"N": N, "reps": reps, "us_per_call": out}}), flush=True)
 
 
 def dft_gemm_compare(reps, cases=((128, 64), (256, 16), (512, 4))):
@@ -459,6 +498,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps({"serving": leg_serving()}), flush=True)
+        return
+    if args.profile_ops:
+        profile_ops(args.N, args.reps)
         return
     import torch
 

@@ -146,64 +146,120 @@ __device__ __forceinline__ float op_forcing_x(const float* th, float t, float x,
   return p.C * expf(-(d * d) / (2.0f * sig * sig));
 }
 
-// Operator-level stencils (NS_solver.py:206-265, 474-477, 621-640), one
-// thread per cell, blockIdx.y = slot.
+// Arithmetic of one cell of the operator-level stencils (NS_solver.py:206-265,
+// 474-477), shared by the one-cell and the four-cell kernels so both give the
+// same bits.  fxm/fxp: x-neighbours of the field differenced along x; fym/fyp
+// and fc: y-neighbours and centre of the field differenced along y (the same
+// field except for the divergence, which takes u_x along x and u_y along y);
+// ub/uc: the advecting velocity at the cell.
+__device__ __forceinline__ void stencil_cell(int op, const nsk_plan& p, float fc, float fxm, float fxp,
+                                             float fym, float fyp, float ub, float uc, float& o0, float& o1) {
+  switch (op) {
+    case 0:  // grad
+      o0 = (fxp - fxm) * p.inv_2dx;
+      o1 = (fyp - fym) * p.inv_2dx;
+      break;
+    case 1:  // laplacian
+      o0 = (fxm + fxp + fyp + fym - 4.0f * fc) * p.inv_dx2;
+      break;
+    case 2: {  // divergence
+      const float gx = (fxp - fxm) * p.inv_2dx;
+      const float gy = (fyp - fym) * p.inv_2dx;
+      o0 = gx + gy;
+      break;
+    }
+    case 3: {  // central advection
+      const float fx = (fxp - fxm) * p.inv_2dx;
+      const float fy = (fyp - fym) * p.inv_2dx;
+      o0 = -(ub * fx + uc * fy);
+      break;
+    }
+    default: {  // 4: upwind advection, backward difference where u >= 0
+      const float dfx = (ub >= 0.0f ? fc - fxm : fxp - fc) * p.inv_dx;
+      const float dfy = (uc >= 0.0f ? fc - fym : fyp - fc) * p.inv_dx;
+      o0 = -(ub * dfx + uc * dfy);
+      break;
+    }
+  }
+}
+
+// Operator-level stencils and the forcing field (NS_solver.py:206-265,
+// 474-477, 621-640), one thread per cell, blockIdx.y = slot.  Used when a
+// buffer is not 16-byte aligned.
 __global__ void k_stencil(StencilArgs a) {
   const int N = a.plan.N;
   const size_t NN = (size_t)N * N;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= NN) return;
   const int x = (int)(i / N), y = (int)(i % N);
-  const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
-  const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
   const long long bi = (long long)blockIdx.y * a.in_bstride;
   const long long bo = (long long)blockIdx.y * a.out_bstride;
   const nsk_plan& p = a.plan;
-#define AT(ptr, xx, yy) (ptr)[bi + (size_t)(xx) * N + (yy)]
-  switch (a.op) {
-    case 0: {  // grad
-      const float* f = a.a;
-      a.out0[bo + i] = (AT(f, xp, y) - AT(f, xm, y)) * p.inv_2dx;
-      a.out1[bo + i] = (AT(f, x, yp) - AT(f, x, ym)) * p.inv_2dx;
-      break;
-    }
-    case 1: {  // laplacian
-      const float* f = a.a;
-      a.out0[bo + i] = (AT(f, xm, y) + AT(f, xp, y) + AT(f, x, yp) + AT(f, x, ym) - 4.0f * AT(f, x, y)) * p.inv_dx2;
-      break;
-    }
-    case 2: {  // divergence of (2,N,N)
-      const float* ux = a.a;
-      const float* uy = a.a + NN;
-      const float gx = (AT(ux, xp, y) - AT(ux, xm, y)) * p.inv_2dx;
-      const float gy = (AT(uy, x, yp) - AT(uy, x, ym)) * p.inv_2dx;
-      a.out0[bo + i] = gx + gy;
-      break;
-    }
-    case 3: {  // central advection of field a.a by (a.b, a.c)
-      const float* f = a.a;
-      const float fx = (AT(f, xp, y) - AT(f, xm, y)) * p.inv_2dx;
-      const float fy = (AT(f, x, yp) - AT(f, x, ym)) * p.inv_2dx;
-      a.out0[bo + i] = -(AT(a.b, x, y) * fx + AT(a.c, x, y) * fy);
-      break;
-    }
-    case 4: {  // upwind advection
-      const float* f = a.a;
-      const float c = AT(f, x, y);
-      const float ux = AT(a.b, x, y), uy = AT(a.c, x, y);
-      const float dfx = (ux >= 0.0f ? c - AT(f, xm, y) : AT(f, xp, y) - c) * p.inv_dx;
-      const float dfy = (uy >= 0.0f ? c - AT(f, x, ym) : AT(f, x, yp) - c) * p.inv_dx;
-      a.out0[bo + i] = -(ux * dfx + uy * dfy);
-      break;
-    }
-    case 5: {  // forcing field (2,N,N)
-      const float xc = p.Lf * ((float)x / (float)N);
-      a.out0[bo + i] = op_forcing_x(a.theta, a.t, xc, p);
-      a.out0[bo + NN + i] = 0.0f;
-      break;
-    }
+  if (a.op == 5) {
+    a.out0[bo + i] = op_forcing_x(a.theta, a.t, p.Lf * ((float)x / (float)N), p);
+    a.out0[bo + NN + i] = 0.0f;
+    return;
   }
-#undef AT
+  const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+  const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
+  const float* F = a.a + bi;
+  const float* G = (a.op == 2 ? a.a + NN : a.a) + bi;
+  const size_t row = (size_t)x * N;
+  const float fc = (a.op == 1 || a.op == 4) ? G[row + y] : 0.0f;
+  const float ub = a.op >= 3 ? a.b[bi + row + y] : 0.0f;
+  const float uc = a.op >= 3 ? a.c[bi + row + y] : 0.0f;
+  float o0 = 0.0f, o1 = 0.0f;
+  stencil_cell(a.op, p, fc, F[(size_t)xm * N + y], F[(size_t)xp * N + y], G[row + ym], G[row + yp], ub, uc,
+               o0, o1);
+  a.out0[bo + i] = o0;
+  if (a.op == 0) a.out1[bo + i] = o1;
+}
+
+// The same operators, four consecutive cells of a row per thread: 16-byte
+// loads of the rows x-1, x, x+1 (and of the velocity), two scalar loads for
+// the y-neighbours past the ends, 16-byte stores; the forcing profile, a
+// function of x only, is evaluated once per four cells.  All buffers
+// 16-byte aligned (checked on the host).
+__global__ void k_stencil4(StencilArgs a) {
+  const int N = a.plan.N, Q = N >> 2;
+  const size_t NN = (size_t)N * N;
+  // XCD-aware: each XCD takes a contiguous range of rows, so the rows x-1
+  // and x+1 a workgroup reads were mostly fetched into the same L2
+  const size_t i4 = (size_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i4 >= NN / 4) return;
+  const int x = (int)(i4 / Q), y0 = (int)(i4 % Q) * 4;
+  const long long bi = (long long)blockIdx.y * a.in_bstride;
+  const long long bo = (long long)blockIdx.y * a.out_bstride;
+  const nsk_plan& p = a.plan;
+  const size_t row = (size_t)x * N;
+  const size_t o = bo + row + y0;
+  if (a.op == 5) {
+    const float v = op_forcing_x(a.theta, a.t, p.Lf * ((float)x / (float)N), p);
+    *(float4*)(a.out0 + o) = make_float4(v, v, v, v);
+    *(float4*)(a.out0 + NN + o) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+  const float* F = a.a + bi;
+  const float* G = (a.op == 2 ? a.a + NN : a.a) + bi;
+  const float4 m4 = *(const float4*)(F + (size_t)xm * N + y0);
+  const float4 p4 = *(const float4*)(F + (size_t)xp * N + y0);
+  const float4 g4 = *(const float4*)(G + row + y0);
+  const float g[6] = {G[row + ((y0 - 1) & (N - 1))], g4.x, g4.y, g4.z, g4.w, G[row + ((y0 + 4) & (N - 1))]};
+  const float fm[4] = {m4.x, m4.y, m4.z, m4.w}, fp[4] = {p4.x, p4.y, p4.z, p4.w};
+  float ub[4] = {0.0f, 0.0f, 0.0f, 0.0f}, uc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (a.op >= 3) {
+    const float4 b4 = *(const float4*)(a.b + bi + row + y0);
+    const float4 c4 = *(const float4*)(a.c + bi + row + y0);
+    ub[0] = b4.x; ub[1] = b4.y; ub[2] = b4.z; ub[3] = b4.w;
+    uc[0] = c4.x; uc[1] = c4.y; uc[2] = c4.z; uc[3] = c4.w;
+  }
+  float r0[4], r1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    stencil_cell(a.op, p, g[k + 1], fm[k], fp[k], g[k], g[k + 2], ub[k], uc[k], r0[k], r1[k]);
+  *(float4*)(a.out0 + o) = make_float4(r0[0], r0[1], r0[2], r0[3]);
+  if (a.op == 0) *(float4*)(a.out1 + o) = make_float4(r1[0], r1[1], r1[2], r1[3]);
 }
 
 // out[b*ncomp + c] = sum_{n < nsteps_b, band} part[b][n][band][c], in a
@@ -347,7 +403,13 @@ int stencil(const nsk_plan* p, int op, const float* a0, const float* b0, const f
   a.theta = theta;
   a.t = t;
   const size_t NN = (size_t)p->N * p->N;
-  hipLaunchKernelGGL(k_stencil, dim3((unsigned)((NN + 255) / 256), p->batch), dim3(256), 0, st, a);
+  auto al16 = [](const void* q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+  const bool vec = al16(a0) && al16(b0) && al16(c0) && al16(o0) && al16(o1) && in_bs % 4 == 0 &&
+                   out_bs % 4 == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_stencil4, dim3((unsigned)((NN / 4 + 255) / 256), p->batch), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_stencil, dim3((unsigned)((NN + 255) / 256), p->batch), dim3(256), 0, st, a);
   HIP_TRY(hipGetLastError());
   return NSK_OK;
 }

@@ -246,4 +246,10 @@ NSK_DECLARE_LAUNCHERS(1024)
 NSK_DECLARE_LAUNCHERS(2048)
 NSK_DECLARE_LAUNCHERS(4096)
 
+// The hardware hands workgroup i to XCD i % 8; give each XCD a contiguous
+// range of rows/bands/columns so that neighbours share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  return (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+}
+
 }  // namespace nsk

@@ -90,11 +90,6 @@ __global__ void k_init_tw() {
   }
 }
 
-// The hardware hands workgroup i to XCD i % 8; give each XCD a contiguous
-// range of bands/columns so that neighbours share that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-  return (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
-}
 
 // GaussianForcingField._x_component in float32, same operation order
 // (NS_solver.py:621-627); rintf is round-half-to-even like jnp.round.
