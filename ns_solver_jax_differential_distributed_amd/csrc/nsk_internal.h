@@ -97,9 +97,30 @@ struct RowGeo;
 #define NSK_ROW_RMIN4096 2
 #endif
 template <> struct RowGeo<4096> { static constexpr int E = NSK_ROW_E4096, LG = 1, RMIN = NSK_ROW_RMIN4096, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
-template <> struct RowGeo<2048> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
-template <> struct RowGeo<1024> { static constexpr int E = 16, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
-template <> struct RowGeo<512> { static constexpr int E = 16, LG = 2, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
+// 8 points per thread at N = 512 and 1024: twice the threads per row, so a
+// launch that cannot fill the GPU with rows (one field, a few slots) gets
+// twice the waves without shorter bands.  Per step (measured): N = 512,
+// one field 34.0 -> 29.0 us, 16 slots 56.9 -> 49.3 us (with 2048 target
+// waves; the gradient configuration, 16 slots x 2000 steps, 0.376 -> 0.319 s),
+// 48 slots 118.6 -> 127 us; N = 1024, one field 36.8 -> 33.6 us, 24 slots
+// 227 -> 220 us, 8 slots 81.7 -> 83.2 us.  N = 2048 loses (4 / 8 slots
+// +13 / +8 %) and keeps 16.  One geometry per N for every batch size keeps
+// a trajectory's bits independent of what it is batched with.
+#ifndef NSK_ROW_E2048
+#define NSK_ROW_E2048 16
+#endif
+#ifndef NSK_ROW_E1024
+#define NSK_ROW_E1024 8
+#endif
+template <> struct RowGeo<2048> { static constexpr int E = NSK_ROW_E2048, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
+template <> struct RowGeo<1024> { static constexpr int E = NSK_ROW_E1024, LG = 1, RMIN = 2, RMAX = 16; static constexpr bool DB = NSK_ROW_DB; };
+#ifndef NSK_ROW_E512
+#define NSK_ROW_E512 8
+#endif
+#ifndef NSK_ROW_LG512
+#define NSK_ROW_LG512 1
+#endif
+template <> struct RowGeo<512> { static constexpr int E = NSK_ROW_E512, LG = NSK_ROW_LG512, RMIN = 2, RMAX = 16; static constexpr bool DB = true; };
 template <> struct RowGeo<256> { static constexpr int E = 16, LG = 4, RMIN = 2, RMAX = 8; static constexpr bool DB = true; };
 template <> struct RowGeo<128> { static constexpr int E = 16, LG = 8, RMIN = 2, RMAX = 4; static constexpr bool DB = true; };
 template <> struct RowGeo<64> { static constexpr int E = 16, LG = 16, RMIN = 2, RMAX = 4; static constexpr bool DB = true; };
@@ -113,6 +134,9 @@ template <> struct RowGeo<8> { static constexpr int E = 16, LG = 8, RMIN = 1, RM
 #ifndef NSK_ROW_WAVES_MID
 #define NSK_ROW_WAVES_MID 2048
 #endif
+#ifndef NSK_ROW_WAVES_512
+#define NSK_ROW_WAVES_512 2048
+#endif
 // Rows per stream for a launch over `batch` slots: enough workgroups to
 // fill the GPU (about 2048 resident waves; 1024 below N = 1024, where a
 // wave holds several streams), as few halo rows as that allows.
@@ -121,7 +145,7 @@ constexpr int pick_rows(int batch, int nrows = N) {
   using G = RowGeo<N>;
   constexpr int EE = N < G::E ? N : G::E;
   constexpr int TL = N / EE;
-  const long target = N >= 4096 ? 2048 : (N >= 1024 ? NSK_ROW_WAVES_MID : 1024);
+  const long target = N >= 4096 ? 2048 : (N >= 1024 ? NSK_ROW_WAVES_MID : (N == 512 ? NSK_ROW_WAVES_512 : 1024));
   // waves = batch * (nrows / rows) * TL / 64
   const long num = (long)batch * nrows * TL;
   int rows = G::RMIN;
