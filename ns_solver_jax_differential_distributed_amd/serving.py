@@ -340,8 +340,11 @@ class NSServer:
     chunks sized by GPU time keep any one launch short."""
 
     # N = 1024 / 2048 at 24 / 8 slots: 7 % more saturated throughput on a
-    # 300-request mix than 16 / 6 (measured); idle slots exit at once
-    DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 48, 1024: 24, 2048: 8, 4096: 2}
+    # 300-request mix than 16 / 6 (measured); idle slots exit at once.
+    # N = 512 at 64 slots: a launch over 48 gets 8-row bands in 3072 waves,
+    # the slower shape of that row pass; 32 or 64 slots get 2048 waves
+    # (the mix 0.349-0.355 s at 48, 0.338 at 32, 0.336 at 64).
+    DEFAULT_CAPACITY = {128: 128, 256: 96, 512: 64, 1024: 24, 2048: 8, 4096: 2}
     SMALL_N = 256
     # steady cell-updates/s of a full launch and the floor of one step's two
     # launches, for sizing chunks (measured on MI355X)
