@@ -66,6 +66,8 @@ def parse():
                     help="only compare the column pass at small N with the same transforms as FP32 GEMMs")
     ap.add_argument("--serving-only", action="store_true",
                     help="only run the serving leg (for a kernel trace of the served mix)")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="only run the gradient leg (for a kernel trace of the gradient configuration)")
     ap.add_argument("--profile-gradient", action="store_true",
                     help="only run rollout + gradient (every state checkpointed) of the workload at --N "
                          "for --steps steps, three times, for a kernel trace of the gradient path")
@@ -510,6 +512,9 @@ def main():
         return
     if args.serving_only:
         print(json.dumps({"serving": leg_serving()}), flush=True)
+        return
+    if args.gradient_only:
+        print(json.dumps({"gradient": leg_gradient()}), flush=True)
         return
     if args.profile_ops:
         profile_ops(args.N, args.reps)
