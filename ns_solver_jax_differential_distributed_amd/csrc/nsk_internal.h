@@ -181,13 +181,41 @@ template <> struct ColGeo<4096> { static constexpr int E = NSK_COL_E4096, W = NS
 #define NSK_COL_W1024 4
 #endif
 #ifndef NSK_COL_W512
-#define NSK_COL_W512 8
+#define NSK_COL_W512 4
 #endif
 template <> struct ColGeo<2048> { static constexpr int E = 16, W = NSK_COL_W2048, NWG = 1024 / NSK_COL_W2048; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<1024> { static constexpr int E = 16, W = NSK_COL_W1024, NWG = 512 / NSK_COL_W1024; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<512> { static constexpr int E = 16, W = NSK_COL_W512, NWG = 256 / NSK_COL_W512; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<256> { static constexpr int E = 16, W = 16, NWG = 8; static constexpr bool PF = false, XY = false; };
-template <> struct ColGeo<128> { static constexpr int E = 16, W = 32, NWG = 2; static constexpr bool PF = false, XY = false; };
+// 8 points per thread in the column pass at N = 256 and 512 (twice the threads per
+// line, half the column pairs per block): per step (measured) N = 512 one
+// field 29.0 -> 20.6 us, 16 slots 49.3 -> 47.3, 48 slots 127 -> 114 (with 4
+// pairs per block; 8 pairs: 23.9 / 47.3 / 121); N = 256 one field 27.7 ->
+// 23.4, 96 slots 62.6 -> 59.7, 256 slots 118 -> 115 (8 pairs; 16: 25.4 /
+// 62.3 / 122).  N = 1024 gains nothing (8 / 24 slots +4 / +1 %) and keeps
+// 16.  N = 128 would gain too (one field 26.3 -> 21.9 us, 128 slots 33.5 ->
+// 30.6 with 16 pairs), but its rounding then takes the Adam (lr = 0.01)
+// study off the FP64-driven path after 4 steps, below the 5 every build
+// must clear; it keeps 16.
+#ifndef NSK_COL_E1024
+#define NSK_COL_E1024 16
+#endif
+#ifndef NSK_COL_E512
+#define NSK_COL_E512 8
+#endif
+template <> struct ColGeo<1024> { static constexpr int E = NSK_COL_E1024, W = NSK_COL_W1024, NWG = 512 / NSK_COL_W1024; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<512> { static constexpr int E = NSK_COL_E512, W = NSK_COL_W512, NWG = 256 / NSK_COL_W512; static constexpr bool PF = false, XY = false; };
+#ifndef NSK_COL_E256
+#define NSK_COL_E256 8
+#endif
+#ifndef NSK_COL_W256
+#define NSK_COL_W256 8
+#endif
+#ifndef NSK_COL_E128
+#define NSK_COL_E128 16
+#endif
+#ifndef NSK_COL_W128
+#define NSK_COL_W128 32
+#endif
+template <> struct ColGeo<256> { static constexpr int E = NSK_COL_E256, W = NSK_COL_W256, NWG = 128 / NSK_COL_W256; static constexpr bool PF = false, XY = false; };
+template <> struct ColGeo<128> { static constexpr int E = NSK_COL_E128, W = NSK_COL_W128, NWG = 64 / NSK_COL_W128; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<64> { static constexpr int E = 16, W = 32, NWG = 1; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<32> { static constexpr int E = 16, W = 16, NWG = 1; static constexpr bool PF = false, XY = false; };
 template <> struct ColGeo<16> { static constexpr int E = 16, W = 8, NWG = 1; static constexpr bool PF = false, XY = false; };
